@@ -1,0 +1,505 @@
+// s512_kernels.hpp -- device kernels of the multiway sample-sort pipeline.
+//
+// Reference mapping.  The reference's Sort512::Sort is a recursive quicksort:
+// pick a pivot (sort512.hpp:5916-5926), 2-way Partition512 (5729-5910),
+// recurse, and sort leaves of <= 16 vectors with SmallSort16V (2255, 5312).
+// Here one level of recursion partitions every open segment at once into
+// F = 2^logf ordinary buckets plus one equality bucket per splitter (so runs of
+// equal keys finish in O(n), where the reference's `<= pivot` partition goes
+// quadratic), and leaves of up to one workgroup tile are sorted by the bitonic
+// tile kernels of s512_tile.hpp.
+//
+// Pipeline of one level (all segments of the level in each launch):
+//   k_sample      draw nsample keys per segment at hashed positions
+//   (tile sort)   sort each segment's sample
+//   k_build_tree  evenly spaced splitters, de-duplicated, Eytzinger tree
+//   k_count       per chunk: classify, LDS histogram -> hist[chunk][bucket]
+//   k_colscan     per segment: prefix over chunks per bucket, bucket starts
+//   k_scatter     per chunk: classify, rank in LDS, stage by bucket in LDS,
+//                 write coalesced runs to the other array
+//   k_plan        route each bucket: done / tile sort class / next level
+#pragma once
+#include "s512_common.hpp"
+#include "s512_tile.hpp"
+
+namespace s512 {
+
+constexpr int FMAX_LOG = 9;                 // at most 512-way fan-out per level
+constexpr int FMAX = 1 << FMAX_LOG;
+constexpr int BMAX = 2 * FMAX;              // ordinary + equality buckets
+constexpr int SCAT_P = 512;                 // scatter workgroup
+constexpr int SCAT_E = 16;                  // keys per thread per round
+constexpr int SCAT_ROUND = SCAT_P * SCAT_E; // keys staged in LDS per round
+constexpr int CNT_P = 256;                  // count workgroup
+constexpr int CNT_U = 16;                   // loads in flight per thread
+constexpr uint32_t COPY_PIECE = 1u << 16;   // keys per copy work item
+
+struct TileSeg {
+    uint64_t begin;
+    uint32_t size;
+    uint32_t buf;
+};
+
+__device__ __forceinline__ uint64_t mix64(uint64_t seed, uint64_t i) {
+    uint64_t z = seed * 0x9E3779B97F4A7C15ull + (i + 1) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// ---------------------------------------------------------------- block scan
+// Exclusive scan of one value per thread; returns the prefix, sets *total.
+template <int P>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wtot, uint32_t* total) {
+    constexpr int NW = P / 64;
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d);
+        if (lane >= (uint32_t)d) x += y;
+    }
+    if (lane == 63) wtot[wid] = x;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        const uint32_t s = wtot[w];
+        off += (w < (int)wid) ? s : 0u;
+        tot += s;
+    }
+    *total = tot;
+    __syncthreads();
+    return off + x - v;
+}
+
+// ---------------------------------------------------------------- classify
+template <class K>
+__device__ __forceinline__ uint32_t classify(const K* tree, const K* srt, int logf, K x) {
+    uint32_t t = 1;
+    for (int l = 0; l < logf; ++l) t = 2 * t + (tree[t] < x ? 1u : 0u);
+    const uint32_t j = t - (1u << logf);     // number of splitters < x
+    return 2 * j + (srt[j] == x ? 1u : 0u);  // odd: equality bucket of splitter j
+}
+
+// ---------------------------------------------------------------- tile kernels
+// Sort every listed segment (size <= P*E) in one workgroup; result goes to the
+// caller's array `a` at the same range.
+template <class T, int P, int E>
+__global__ __launch_bounds__(P) void k_tile_sort(const TileSeg* segs, const uint32_t* count, T* a, const T* b) {
+    using Tr = KeyTraits<T>;
+    using K = typename Tr::K;
+    using BB = BlockBitonic<K, P, E>;
+    __shared__ K lds[BB::N];
+    const uint32_t cnt = *count;
+    const uint32_t t = threadIdx.x;
+    for (uint32_t s = blockIdx.x; s < cnt; s += gridDim.x) {
+        const TileSeg sg = segs[s];
+        const T* src = (sg.buf ? b : a) + sg.begin;
+        T* dst = a + sg.begin;
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const uint32_t i = m * P + t;
+            lds[BB::phys(i)] = i < sg.size ? Tr::enc(src[i]) : Tr::KMAX;
+        }
+        K v[E];
+        BB::from_striped(v, lds, t);
+        BB::sort(v, lds, t);
+        BB::to_striped(v, lds, t);
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const uint32_t i = m * P + t;
+            if (i < sg.size) dst[i] = Tr::dec(lds[BB::phys(i)]);
+        }
+        __syncthreads();
+    }
+}
+
+// One wave per segment (size <= 64*E), four waves per workgroup.
+template <class T, int E>
+__global__ __launch_bounds__(256) void k_wave_sort(const TileSeg* segs, const uint32_t* count, T* a, const T* b) {
+    using Tr = KeyTraits<T>;
+    using K = typename Tr::K;
+    const uint32_t cnt = *count;
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint32_t s = blockIdx.x * 4 + (threadIdx.x >> 6); s < cnt; s += gridDim.x * 4) {
+        const TileSeg sg = segs[s];
+        const T* src = (sg.buf ? b : a) + sg.begin;
+        T* dst = a + sg.begin;
+        K v[E];
+#pragma unroll
+        for (int r = 0; r < E; ++r) {
+            const uint32_t i = lane * E + r;
+            v[r] = i < sg.size ? Tr::enc(src[i]) : Tr::KMAX;
+        }
+        WaveBitonic<K, E>::sort(v, lane);
+#pragma unroll
+        for (int r = 0; r < E; ++r) {
+            const uint32_t i = lane * E + r;
+            if (i < sg.size) dst[i] = Tr::dec(v[r]);
+        }
+    }
+}
+
+// Key/value tiles: sort (encoded key << 32 | position) as one 64-bit key, then
+// gather the values by position.  Pads (all ones) sort after every real pair,
+// including real pairs whose key is INT_MAX.
+__device__ __forceinline__ uint64_t kv_pack(int32_t k, uint32_t pos) {
+    return ((uint64_t)KeyTraits<int32_t>::enc(k) << 32) | pos;
+}
+
+template <int P, int E>
+__global__ __launch_bounds__(P) void k_tile_sort_kv(const TileSeg* segs, const uint32_t* count, int32_t* ka,
+                                                    int32_t* va, const int32_t* kb, const int32_t* vb) {
+    using BB = BlockBitonic<uint64_t, P, E>;
+    __shared__ uint64_t lds[BB::N];
+    const uint32_t cnt = *count;
+    const uint32_t t = threadIdx.x;
+    for (uint32_t s = blockIdx.x; s < cnt; s += gridDim.x) {
+        const TileSeg sg = segs[s];
+        const int32_t* sk = (sg.buf ? kb : ka) + sg.begin;
+        const int32_t* sv = (sg.buf ? vb : va) + sg.begin;
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const uint32_t i = m * P + t;
+            lds[BB::phys(i)] = i < sg.size ? kv_pack(sk[i], i) : ~0ull;
+        }
+        uint64_t v[E];
+        BB::from_striped(v, lds, t);
+        BB::sort(v, lds, t);
+        BB::to_striped(v, lds, t);
+        int32_t ok[E], ov[E];
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const uint32_t i = m * P + t;
+            if (i < sg.size) {
+                const uint64_t c = lds[BB::phys(i)];
+                ok[m] = KeyTraits<int32_t>::dec((uint32_t)(c >> 32));
+                ov[m] = sv[(uint32_t)c];
+            }
+        }
+        __syncthreads();  // every value gathered before any is overwritten
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const uint32_t i = m * P + t;
+            if (i < sg.size) {
+                ka[sg.begin + i] = ok[m];
+                va[sg.begin + i] = ov[m];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <int E>
+__global__ __launch_bounds__(256) void k_wave_sort_kv(const TileSeg* segs, const uint32_t* count, int32_t* ka,
+                                                      int32_t* va, const int32_t* kb, const int32_t* vb) {
+    const uint32_t cnt = *count;
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint32_t s = blockIdx.x * 4 + (threadIdx.x >> 6); s < cnt; s += gridDim.x * 4) {
+        const TileSeg sg = segs[s];
+        const int32_t* sk = (sg.buf ? kb : ka) + sg.begin;
+        const int32_t* sv = (sg.buf ? vb : va) + sg.begin;
+        uint64_t v[E];
+#pragma unroll
+        for (int r = 0; r < E; ++r) {
+            const uint32_t i = lane * E + r;
+            v[r] = i < sg.size ? kv_pack(sk[i], i) : ~0ull;
+        }
+        WaveBitonic<uint64_t, E>::sort(v, lane);
+        int32_t ov[E];
+#pragma unroll
+        for (int r = 0; r < E; ++r) {
+            const uint32_t i = lane * E + r;
+            if (i < sg.size) ov[r] = sv[(uint32_t)v[r]];
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int r = 0; r < E; ++r) {
+            const uint32_t i = lane * E + r;
+            if (i < sg.size) {
+                ka[sg.begin + i] = KeyTraits<int32_t>::dec((uint32_t)(v[r] >> 32));
+                va[sg.begin + i] = ov[r];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- sampling
+template <class T>
+__global__ __launch_bounds__(256) void k_sample(const PartSeg* ps, const T* a, const T* b,
+                                                typename KeyTraits<T>::K* samples, uint32_t smax, uint64_t seed) {
+    using Tr = KeyTraits<T>;
+    const PartSeg s = ps[blockIdx.x];
+    const T* src = (s.buf ? b : a) + s.begin;
+    typename Tr::K* out = samples + (uint64_t)blockIdx.x * smax;
+    for (uint32_t i = threadIdx.x; i < s.nsample; i += blockDim.x) {
+        const uint64_t h = mix64(seed ^ (s.begin * 0x632BE59BD9B4E019ull), i);
+        const uint64_t pos = __umul64hi(h, s.size);  // uniform in [0, size)
+        out[i] = Tr::enc(src[pos]);
+    }
+}
+
+// Splitters at evenly spaced ranks of the sorted sample; duplicates removed;
+// padded with KMAX to F-1 entries; laid out as an implicit (Eytzinger) tree.
+// trees[tree_base + 0 .. F)   : tree nodes 1..F-1 (slot 0 unused)
+// trees[tree_base + F .. 2F)  : sorted unique splitters, srt[F-1] = KMAX
+template <class K>
+__global__ __launch_bounds__(FMAX) void k_build_tree(const PartSeg* ps, const K* samples, uint32_t smax, K* trees) {
+    __shared__ K cand[FMAX];
+    __shared__ uint32_t wtot[FMAX / 64];
+    const PartSeg s = ps[blockIdx.x];
+    const uint32_t F = 1u << s.logf;
+    const K* smp = samples + (uint64_t)blockIdx.x * smax;
+    const uint32_t t = threadIdx.x;
+    K c = (K)~(K)0;
+    bool keep = false;
+    if (t + 1 < F) {
+        const uint32_t r = (uint32_t)(((uint64_t)(t + 1) * s.nsample) / F);
+        c = smp[r > 0 ? r - 1 : 0];
+        keep = true;
+        if (t > 0) {
+            const uint32_t rp = (uint32_t)(((uint64_t)t * s.nsample) / F);
+            keep = smp[rp > 0 ? rp - 1 : 0] != c;
+        }
+    }
+    uint32_t nuniq;
+    const uint32_t pos = block_excl_scan<FMAX>(keep ? 1u : 0u, wtot, &nuniq);
+    K* tree = trees + s.tree_base;
+    K* srt = tree + F;
+    if (t < F) srt[t] = (K)~(K)0;
+    __syncthreads();
+    if (keep) srt[pos] = c;
+    __syncthreads();
+    // copy to LDS, then fill the tree: node n at depth d, rank p in its level,
+    // holds the sorted element of in-order index ((2p+1) << (logf-1-d)) - 1.
+    if (t < F) cand[t] = srt[t];
+    __syncthreads();
+    if (t >= 1 && t < F) {
+        const int d = 31 - __clz(t);
+        const uint32_t p = t - (1u << d);
+        tree[t] = cand[((2 * p + 1) << (s.logf - 1 - d)) - 1];
+    }
+    if (t == 0) tree[0] = (K)~(K)0;
+}
+
+// ---------------------------------------------------------------- count
+template <class T>
+__global__ __launch_bounds__(CNT_P) void k_count(const PartSeg* ps, const ChunkRef* cr, const T* a, const T* b,
+                                                 const typename KeyTraits<T>::K* trees, uint32_t* hist) {
+    using Tr = KeyTraits<T>;
+    using K = typename Tr::K;
+    __shared__ K ltree[FMAX];
+    __shared__ K lsrt[FMAX];
+    __shared__ uint32_t lh[BMAX];
+    const ChunkRef c = cr[blockIdx.x];
+    const PartSeg s = ps[c.seg];
+    const uint32_t F = 1u << s.logf, B = 2 * F;
+    const uint32_t t = threadIdx.x;
+    for (uint32_t j = t; j < F; j += CNT_P) {
+        ltree[j] = trees[s.tree_base + j];
+        lsrt[j] = trees[s.tree_base + F + j];
+    }
+    for (uint32_t j = t; j < B; j += CNT_P) lh[j] = 0;
+    __syncthreads();
+    const T* src = (s.buf ? b : a) + s.begin;
+    const uint64_t lo = (uint64_t)c.idx * s.chunk;
+    const uint64_t hi = lo + s.chunk < s.size ? lo + s.chunk : s.size;
+    const int logf = (int)s.logf;
+    for (uint64_t base = lo; base < hi; base += (uint64_t)CNT_P * CNT_U) {
+        K x[CNT_U];
+#pragma unroll
+        for (int u = 0; u < CNT_U; ++u) {
+            const uint64_t i = base + (uint64_t)u * CNT_P + t;
+            x[u] = i < hi ? Tr::enc(src[i]) : (K)0;
+        }
+#pragma unroll
+        for (int u = 0; u < CNT_U; ++u) {
+            const uint64_t i = base + (uint64_t)u * CNT_P + t;
+            if (i < hi) atomicAdd(&lh[classify(ltree, lsrt, logf, x[u])], 1u);
+        }
+    }
+    __syncthreads();
+    uint32_t* h = hist + s.hist_base + (uint64_t)c.idx * B;
+    for (uint32_t j = t; j < B; j += CNT_P) h[j] = lh[j];
+}
+
+// ---------------------------------------------------------------- column scan
+// hist[chunk][bucket] -> prefix over earlier chunks of the same bucket;
+// bstart[b] = first position of bucket b inside the segment; bsize[b] = size.
+__global__ __launch_bounds__(BMAX) void k_colscan(const PartSeg* ps, uint32_t* hist, uint64_t* bstart, uint64_t* bsize) {
+    __shared__ uint32_t wtot[BMAX / 64];
+    const PartSeg s = ps[blockIdx.x];
+    const uint32_t B = 2u << s.logf;
+    const uint32_t t = threadIdx.x;
+    uint32_t run = 0;
+    if (t < B) {
+        uint32_t* h = hist + s.hist_base + t;
+        for (uint32_t c = 0; c < s.nchunks; ++c) {
+            const uint32_t v = h[(uint64_t)c * B];
+            h[(uint64_t)c * B] = run;
+            run += v;
+        }
+    }
+    uint32_t total;
+    const uint32_t start = block_excl_scan<BMAX>(run, wtot, &total);
+    if (t < B) {
+        bstart[s.bkt_base + t] = start;
+        bsize[s.bkt_base + t] = run;
+    }
+}
+
+// ---------------------------------------------------------------- scatter
+// Keys (and values when V != void-like) move from array buf to array 1-buf.
+template <class T, bool KV>
+__global__ __launch_bounds__(SCAT_P) void k_scatter(const PartSeg* ps, const ChunkRef* cr, T* a, T* b,
+                                                    int32_t* va, int32_t* vb,
+                                                    const typename KeyTraits<T>::K* trees, const uint32_t* hist,
+                                                    const uint64_t* bstart) {
+    using Tr = KeyTraits<T>;
+    using K = typename Tr::K;
+    __shared__ K ltree[FMAX];
+    __shared__ K lsrt[FMAX];
+    __shared__ uint32_t cursor[BMAX];
+    __shared__ uint32_t rh[BMAX];
+    __shared__ uint32_t rs[BMAX];
+    __shared__ uint32_t wtot[SCAT_P / 64];
+    __shared__ T stg[SCAT_ROUND];
+    __shared__ uint16_t stb[SCAT_ROUND];
+    __shared__ int32_t stv[KV ? SCAT_ROUND : 1];
+
+    const ChunkRef c = cr[blockIdx.x];
+    const PartSeg s = ps[c.seg];
+    const uint32_t F = 1u << s.logf, B = 2 * F;
+    const uint32_t t = threadIdx.x;
+    for (uint32_t j = t; j < F; j += SCAT_P) {
+        ltree[j] = trees[s.tree_base + j];
+        lsrt[j] = trees[s.tree_base + F + j];
+    }
+    const uint32_t* h = hist + s.hist_base + (uint64_t)c.idx * B;
+    for (uint32_t j = t; j < B; j += SCAT_P) cursor[j] = (uint32_t)bstart[s.bkt_base + j] + h[j];
+    const T* src = (s.buf ? b : a) + s.begin;
+    T* dst = (s.buf ? a : b) + s.begin;
+    const int32_t* vsrc = KV ? ((s.buf ? vb : va) + s.begin) : nullptr;
+    int32_t* vdst = KV ? ((s.buf ? va : vb) + s.begin) : nullptr;
+    const uint64_t lo = (uint64_t)c.idx * s.chunk;
+    const uint64_t hi = lo + s.chunk < s.size ? lo + s.chunk : s.size;
+    const int logf = (int)s.logf;
+
+    for (uint64_t base = lo; base < hi; base += SCAT_ROUND) {
+        const uint32_t cnt = (uint32_t)((hi - base) < (uint64_t)SCAT_ROUND ? (hi - base) : (uint64_t)SCAT_ROUND);
+        for (uint32_t j = t; j < B; j += SCAT_P) rh[j] = 0;
+        __syncthreads();
+        T x[SCAT_E];
+        int32_t xv[SCAT_E];
+        uint32_t bk[SCAT_E], lr[SCAT_E];
+#pragma unroll
+        for (int e = 0; e < SCAT_E; ++e) {
+            const uint32_t i = e * SCAT_P + t;
+            if (i < cnt) {
+                x[e] = src[base + i];
+                if constexpr (KV) xv[e] = vsrc[base + i];
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < SCAT_E; ++e) {
+            const uint32_t i = e * SCAT_P + t;
+            if (i < cnt) {
+                bk[e] = classify(ltree, lsrt, logf, Tr::enc(x[e]));
+                lr[e] = atomicAdd(&rh[bk[e]], 1u);
+            }
+        }
+        __syncthreads();
+        // exclusive scan of the round histogram: two buckets per thread
+        {
+            const uint32_t j0 = 2 * t, j1 = 2 * t + 1;
+            const uint32_t h0 = j0 < B ? rh[j0] : 0u, h1 = j1 < B ? rh[j1] : 0u;
+            uint32_t tot;
+            const uint32_t p = block_excl_scan<SCAT_P>(h0 + h1, wtot, &tot);
+            if (j0 < B) rs[j0] = p;
+            if (j1 < B) rs[j1] = p + h0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < SCAT_E; ++e) {
+            const uint32_t i = e * SCAT_P + t;
+            if (i < cnt) {
+                const uint32_t p = rs[bk[e]] + lr[e];
+                stg[p] = x[e];
+                stb[p] = (uint16_t)bk[e];
+                if constexpr (KV) stv[p] = xv[e];
+            }
+        }
+        __syncthreads();
+        for (uint32_t i = t; i < cnt; i += SCAT_P) {
+            const uint32_t bkt = stb[i];
+            const uint64_t d = (uint64_t)cursor[bkt] + (i - rs[bkt]);
+            dst[d] = stg[i];
+            if constexpr (KV) vdst[d] = stv[i];
+        }
+        __syncthreads();
+        for (uint32_t j = t; j < B; j += SCAT_P) cursor[j] += rh[j];
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- plan
+struct PlanLists {
+    TileSeg* tile[8];         // per tile class
+    uint32_t* tile_count;     // [8]
+    uint32_t tile_cap[8];     // class capacities (keys)
+    uint32_t nclass;
+    TileSeg* copy;            // pieces to copy scratch -> caller's array
+    uint32_t* copy_count;
+    Seg* next;                // segments for the next partition level
+    uint32_t* next_count;
+};
+
+__global__ __launch_bounds__(256) void k_plan(const PartSeg* ps, const uint64_t* bstart, const uint64_t* bsize,
+                                              PlanLists pl) {
+    const PartSeg s = ps[blockIdx.x];
+    const uint32_t B = 2u << s.logf;
+    const uint32_t b = blockIdx.y * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    const uint64_t sz = bsize[s.bkt_base + b];
+    if (sz == 0) return;
+    const uint64_t begin = s.begin + bstart[s.bkt_base + b];
+    const uint32_t buf = 1u - s.buf;
+    if ((b & 1u) || sz == 1) {
+        if (buf == 1) {
+            const uint32_t np = (uint32_t)((sz + COPY_PIECE - 1) / COPY_PIECE);
+            const uint32_t o = atomicAdd(pl.copy_count, np);
+            for (uint32_t p = 0; p < np; ++p) {
+                const uint64_t pb = (uint64_t)p * COPY_PIECE;
+                pl.copy[o + p] = TileSeg{begin + pb, (uint32_t)((sz - pb) < COPY_PIECE ? (sz - pb) : COPY_PIECE), 1u};
+            }
+        }
+        return;
+    }
+    for (uint32_t k = 0; k < pl.nclass; ++k) {
+        if (sz <= pl.tile_cap[k]) {
+            const uint32_t o = atomicAdd(&pl.tile_count[k], 1u);
+            pl.tile[k][o] = TileSeg{begin, (uint32_t)sz, buf};
+            return;
+        }
+    }
+    const uint32_t o = atomicAdd(pl.next_count, 1u);
+    pl.next[o] = Seg{begin, sz, buf, 0u};
+}
+
+// ---------------------------------------------------------------- copy
+template <class T, bool KV>
+__global__ __launch_bounds__(256) void k_copy_back(const TileSeg* pieces, const uint32_t* count, T* a, const T* b,
+                                                   int32_t* va, const int32_t* vb) {
+    const uint32_t cnt = *count;
+    for (uint32_t p = blockIdx.x; p < cnt; p += gridDim.x) {
+        const TileSeg sg = pieces[p];
+        for (uint32_t i = threadIdx.x; i < sg.size; i += blockDim.x) {
+            a[sg.begin + i] = b[sg.begin + i];
+            if constexpr (KV) va[sg.begin + i] = vb[sg.begin + i];
+        }
+    }
+}
+
+}  // namespace s512

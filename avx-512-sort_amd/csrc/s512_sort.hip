@@ -1,0 +1,356 @@
+// s512_sort.hip -- the sort engine (multiway sample sort + bitonic tiles) and
+// its C entry points: Sort512::Sort (sort512.hpp:5960) and Sort512kv::Sort
+// (sort512kv.hpp:6132).
+//
+// Host loop, one iteration per partition level:
+//   1. describe every open segment (fan-out, sample size, chunks, offsets)
+//   2. upload the descriptors, run sample -> sample tile sort -> tree ->
+//      count -> column scan -> scatter -> plan on the stream
+//   3. read back the plan counters; launch the copy-back and the tile sorts of
+//      finished buckets; read back the segments that need another level.
+// Tile sorts of one level overlap the partition kernels of the next (they work
+// on disjoint ranges and are ordered on the same stream).
+#include "s512_kernels.hpp"
+#include "s512_runtime.hpp"
+#include "s512_engine.hpp"
+#include <algorithm>
+
+namespace s512 {
+
+// ---------------------------------------------------------------- tile classes
+static const uint32_t kCaps32[] = {256, 512, 1024, 2048, 4096, 8192, 16384};
+static const uint32_t kCaps64[] = {256, 512, 1024, 2048, 4096, 8192};
+
+template <class T>
+static void launch_tile_keys(int cls, const TileSeg* segs, const uint32_t* dcount, uint32_t count, T* a,
+                             const T* b, hipStream_t st) {
+    using K = typename KeyTraits<T>::K;
+    if (count == 0) return;
+    const uint32_t gw = (count + 3) / 4;
+    if constexpr (sizeof(K) == 4) {
+        switch (cls) {
+            case 0: k_wave_sort<T, 4><<<gw, 256, 0, st>>>(segs, dcount, a, b); break;
+            case 1: k_wave_sort<T, 8><<<gw, 256, 0, st>>>(segs, dcount, a, b); break;
+            case 2: k_tile_sort<T, 128, 8><<<count, 128, 0, st>>>(segs, dcount, a, b); break;
+            case 3: k_tile_sort<T, 256, 8><<<count, 256, 0, st>>>(segs, dcount, a, b); break;
+            case 4: k_tile_sort<T, 256, 16><<<count, 256, 0, st>>>(segs, dcount, a, b); break;
+            case 5: k_tile_sort<T, 512, 16><<<count, 512, 0, st>>>(segs, dcount, a, b); break;
+            default: k_tile_sort<T, 1024, 16><<<count, 1024, 0, st>>>(segs, dcount, a, b); break;
+        }
+    } else {
+        switch (cls) {
+            case 0: k_wave_sort<T, 4><<<gw, 256, 0, st>>>(segs, dcount, a, b); break;
+            case 1: k_tile_sort<T, 128, 4><<<count, 128, 0, st>>>(segs, dcount, a, b); break;
+            case 2: k_tile_sort<T, 128, 8><<<count, 128, 0, st>>>(segs, dcount, a, b); break;
+            case 3: k_tile_sort<T, 256, 8><<<count, 256, 0, st>>>(segs, dcount, a, b); break;
+            case 4: k_tile_sort<T, 256, 16><<<count, 256, 0, st>>>(segs, dcount, a, b); break;
+            default: k_tile_sort<T, 512, 16><<<count, 512, 0, st>>>(segs, dcount, a, b); break;
+        }
+    }
+}
+
+static void launch_tile_kv(int cls, const TileSeg* segs, const uint32_t* dcount, uint32_t count, int32_t* ka,
+                           int32_t* va, const int32_t* kb, const int32_t* vb, hipStream_t st) {
+    if (count == 0) return;
+    const uint32_t gw = (count + 3) / 4;
+    switch (cls) {
+        case 0: k_wave_sort_kv<4><<<gw, 256, 0, st>>>(segs, dcount, ka, va, kb, vb); break;
+        case 1: k_tile_sort_kv<128, 4><<<count, 128, 0, st>>>(segs, dcount, ka, va, kb, vb); break;
+        case 2: k_tile_sort_kv<128, 8><<<count, 128, 0, st>>>(segs, dcount, ka, va, kb, vb); break;
+        case 3: k_tile_sort_kv<256, 8><<<count, 256, 0, st>>>(segs, dcount, ka, va, kb, vb); break;
+        case 4: k_tile_sort_kv<256, 16><<<count, 256, 0, st>>>(segs, dcount, ka, va, kb, vb); break;
+        default: k_tile_sort_kv<512, 16><<<count, 512, 0, st>>>(segs, dcount, ka, va, kb, vb); break;
+    }
+}
+
+template <class T, bool KV>
+struct TileCfg {
+    static constexpr bool WIDE = KV || sizeof(T) == 8;
+    static const uint32_t* caps() { return WIDE ? kCaps64 : kCaps32; }
+    static int nclass() { return WIDE ? 6 : 7; }
+    static uint32_t maxcap() { return caps()[nclass() - 1]; }
+    static int cls_of(uint64_t n) {
+        for (int c = 0; c < nclass(); ++c)
+            if (n <= caps()[c]) return c;
+        return -1;
+    }
+};
+
+template <class T, bool KV>
+static void launch_tile(int cls, const TileSeg* segs, const uint32_t* dcount, uint32_t count, T* a, int32_t* va,
+                        const T* b, const int32_t* vb, hipStream_t st) {
+    if constexpr (KV) launch_tile_kv(cls, segs, dcount, count, a, va, b, vb, st);
+    else launch_tile_keys<T>(cls, segs, dcount, count, a, b, st);
+}
+
+static inline int ceil_log2(uint64_t x) {
+    int l = 0;
+    while ((1ull << l) < x) ++l;
+    return l;
+}
+
+// ---------------------------------------------------------------- engine
+template <class T, bool KV>
+s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const EngineParams& prm, EngineStats* stats) {
+    using K = typename KeyTraits<T>::K;
+    using TC = TileCfg<T, KV>;
+    if (n <= 1) return S512_OK;
+    if (n >= (1ull << 32)) {
+        set_error("n = %zu exceeds the supported maximum of 2^32-1 keys", n);
+        return S512_ERR_TOO_LARGE;
+    }
+    int dev;
+    s512_status rc = current_device(&dev);
+    if (rc) return rc;
+    DeviceState& ds = device_state(dev);
+    std::lock_guard<std::mutex> lk(ds.mu);
+
+    const int ncls = TC::nclass();
+    // ---- one tile: no partition level at all
+    if (n <= TC::maxcap()) {
+        SizeCounter sc;
+        sc.take<TileSeg>(1);
+        sc.take<uint32_t>(1);
+        if ((rc = ds.arena.reserve(sc.off, st))) return rc;
+        if ((rc = ds.pinned.reserve(sc.off, st))) return rc;
+        Arena ar{(char*)ds.arena.ptr, 0}, hp{(char*)ds.pinned.ptr, 0};
+        TileSeg* dseg = ar.take<TileSeg>(1);
+        uint32_t* dcnt = ar.take<uint32_t>(1);
+        TileSeg* hseg = hp.take<TileSeg>(1);
+        uint32_t* hcnt = hp.take<uint32_t>(1);
+        *hseg = TileSeg{0, (uint32_t)n, 0};
+        *hcnt = 1;
+        S512_HIP(hipMemcpyAsync(ds.arena.ptr, ds.pinned.ptr, sc.off, hipMemcpyHostToDevice, st));
+        launch_tile<T, KV>(TC::cls_of(n), dseg, dcnt, 1, a, va, a, va, st);
+        S512_CHECK_LAUNCH();
+        S512_HIP(hipStreamSynchronize(st));  // pinned staging is reused by the next call
+        if (stats) { stats->levels = 0; stats->tiles = 1; }
+        return S512_OK;
+    }
+
+    // ---- scratch arrays (same size as the input)
+    const size_t dbytes = n * sizeof(T) + (KV ? n * sizeof(int32_t) : 0);
+    if ((rc = ds.data.reserve(dbytes, st))) return rc;
+    T* b = (T*)ds.data.ptr;
+    int32_t* vb = KV ? (int32_t*)((char*)ds.data.ptr + n * sizeof(T)) : nullptr;
+
+    std::vector<Seg> segs{Seg{0, n, 0, 0}};
+    std::vector<PartSeg> ps;
+    std::vector<ChunkRef> crs;
+    const uint32_t smax = 8192;
+    int level = 0;
+    uint64_t tiles = 0;
+    while (!segs.empty()) {
+        if (++level > prm.max_levels) {
+            set_error("partition did not converge within %d levels", prm.max_levels);
+            return S512_ERR_INTERNAL;
+        }
+        const uint32_t m = (uint32_t)segs.size();
+        uint64_t total = 0;
+        for (auto& s : segs) total += s.size;
+        uint64_t chunk = (total + prm.target_chunks - 1) / prm.target_chunks;
+        chunk = (chunk + SCAT_ROUND - 1) / SCAT_ROUND * SCAT_ROUND;
+        chunk = std::max<uint64_t>(SCAT_ROUND, std::min<uint64_t>(chunk, prm.max_chunk));
+        ps.resize(m);
+        crs.clear();
+        uint64_t histN = 0, sumB = 0, maxB = 0;
+        for (uint32_t i = 0; i < m; ++i) {
+            PartSeg& p = ps[i];
+            p.begin = segs[i].begin;
+            p.size = segs[i].size;
+            p.buf = segs[i].buf;
+            int lf = ceil_log2((p.size + prm.target_leaf - 1) / prm.target_leaf);
+            lf = std::max(1, std::min(lf, FMAX_LOG));
+            p.logf = (uint32_t)lf;
+            const uint32_t F = 1u << lf, B = 2 * F;
+            p.nsample = std::min<uint32_t>(smax, prm.oversample * F);
+            p.chunk = chunk;
+            p.nchunks = (uint32_t)((p.size + chunk - 1) / chunk);
+            p.hist_base = histN;
+            p.tree_base = sumB;
+            p.bkt_base = sumB;
+            histN += (uint64_t)p.nchunks * B;
+            sumB += B;
+            maxB = std::max<uint64_t>(maxB, B);
+            for (uint32_t c = 0; c < p.nchunks; ++c) crs.push_back(ChunkRef{i, c});
+        }
+        const uint32_t tc = (uint32_t)crs.size();
+        const uint64_t copy_cap = sumB + total / COPY_PIECE + 1;
+        // ---- arena layout: uploaded prefix first
+        SizeCounter sc;
+        sc.take<PartSeg>(m);
+        sc.take<ChunkRef>(tc);
+        sc.take<TileSeg>(m);
+        sc.take<uint32_t>(32);
+        const size_t up_bytes = sc.off;
+        sc.take<K>((size_t)m * smax);
+        sc.take<K>(sumB);
+        sc.take<uint32_t>(histN);
+        sc.take<uint64_t>(sumB);
+        sc.take<uint64_t>(sumB);
+        for (int c = 0; c < ncls; ++c) sc.take<TileSeg>(sumB);
+        sc.take<TileSeg>(copy_cap);
+        sc.take<Seg>(sumB);
+        if ((rc = ds.arena.reserve(sc.off, st))) return rc;
+        if ((rc = ds.pinned.reserve(up_bytes, st))) return rc;
+        if ((rc = ds.pinned2.reserve(std::max<size_t>(sumB * sizeof(Seg), 4096), st))) return rc;
+
+        Arena ar{(char*)ds.arena.ptr, 0}, hp{(char*)ds.pinned.ptr, 0};
+        PartSeg* d_ps = ar.take<PartSeg>(m);
+        ChunkRef* d_cr = ar.take<ChunkRef>(tc);
+        TileSeg* d_sl = ar.take<TileSeg>(m);
+        uint32_t* d_cnt = ar.take<uint32_t>(32);
+        K* d_smp = ar.take<K>((size_t)m * smax);
+        K* d_tree = ar.take<K>(sumB);
+        uint32_t* d_hist = ar.take<uint32_t>(histN);
+        uint64_t* d_bstart = ar.take<uint64_t>(sumB);
+        uint64_t* d_bsize = ar.take<uint64_t>(sumB);
+        PlanLists pl;
+        memset(&pl, 0, sizeof(pl));
+        for (int c = 0; c < ncls; ++c) {
+            pl.tile[c] = ar.take<TileSeg>(sumB);
+            pl.tile_cap[c] = TC::caps()[c];
+        }
+        pl.nclass = (uint32_t)ncls;
+        pl.tile_count = d_cnt;
+        pl.copy = ar.take<TileSeg>(copy_cap);
+        pl.copy_count = d_cnt + 8;
+        pl.next = ar.take<Seg>(sumB);
+        pl.next_count = d_cnt + 9;
+
+        PartSeg* h_ps = hp.take<PartSeg>(m);
+        ChunkRef* h_cr = hp.take<ChunkRef>(tc);
+        TileSeg* h_sl = hp.take<TileSeg>(m);
+        uint32_t* h_cnt = hp.take<uint32_t>(32);
+        memcpy(h_ps, ps.data(), m * sizeof(PartSeg));
+        memcpy(h_cr, crs.data(), tc * sizeof(ChunkRef));
+        for (uint32_t i = 0; i < m; ++i) h_sl[i] = TileSeg{(uint64_t)i * smax, ps[i].nsample, 0};
+        memset(h_cnt, 0, 32 * sizeof(uint32_t));
+        h_cnt[10] = m;
+        S512_HIP(hipMemcpyAsync(ds.arena.ptr, ds.pinned.ptr, up_bytes, hipMemcpyHostToDevice, st));
+
+        const uint64_t seed = prm.seed + 0x9E37ull * (uint64_t)level;
+        k_sample<T><<<m, 256, 0, st>>>(d_ps, a, b, d_smp, smax, seed);
+        S512_CHECK_LAUNCH();
+        k_tile_sort<K, 512, 16><<<m, 512, 0, st>>>(d_sl, d_cnt + 10, d_smp, d_smp);
+        S512_CHECK_LAUNCH();
+        k_build_tree<K><<<m, FMAX, 0, st>>>(d_ps, d_smp, smax, d_tree);
+        S512_CHECK_LAUNCH();
+        k_count<T><<<tc, CNT_P, 0, st>>>(d_ps, d_cr, a, b, d_tree, d_hist);
+        S512_CHECK_LAUNCH();
+        k_colscan<<<m, BMAX, 0, st>>>(d_ps, d_hist, d_bstart, d_bsize);
+        S512_CHECK_LAUNCH();
+        k_scatter<T, KV><<<tc, SCAT_P, 0, st>>>(d_ps, d_cr, a, b, va, vb, d_tree, d_hist, d_bstart);
+        S512_CHECK_LAUNCH();
+        k_plan<<<dim3(m, (uint32_t)((maxB + 255) / 256)), 256, 0, st>>>(d_ps, d_bstart, d_bsize, pl);
+        S512_CHECK_LAUNCH();
+        uint32_t* h_out = (uint32_t*)ds.pinned2.ptr;
+        S512_HIP(hipMemcpyAsync(h_out, d_cnt, 32 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        S512_HIP(hipStreamSynchronize(st));
+        uint32_t counts[32];
+        memcpy(counts, h_out, sizeof(counts));
+        if (counts[8]) {
+            k_copy_back<T, KV><<<std::min<uint32_t>(counts[8], 16384), 256, 0, st>>>(pl.copy, pl.copy_count, a, b, va,
+                                                                                     vb);
+            S512_CHECK_LAUNCH();
+        }
+        for (int c = 0; c < ncls; ++c) {
+            if (!counts[c]) continue;
+            launch_tile<T, KV>(c, pl.tile[c], d_cnt + c, counts[c], a, va, b, vb, st);
+            S512_CHECK_LAUNCH();
+            tiles += counts[c];
+        }
+        const uint32_t nn = counts[9];
+        segs.clear();
+        if (nn) {
+            if (nn > sumB) {
+                set_error("plan produced %u segments for %llu buckets", nn, (unsigned long long)sumB);
+                return S512_ERR_INTERNAL;
+            }
+            Seg* h_next = (Seg*)ds.pinned2.ptr;
+            S512_HIP(hipMemcpyAsync(h_next, pl.next, nn * sizeof(Seg), hipMemcpyDeviceToHost, st));
+            S512_HIP(hipStreamSynchronize(st));
+            segs.assign(h_next, h_next + nn);
+        }
+    }
+    S512_HIP(hipStreamSynchronize(st));
+    if (stats) { stats->levels = level; stats->tiles = tiles; }
+    return S512_OK;
+}
+
+template s512_status sort_engine<int32_t, false>(int32_t*, int32_t*, size_t, hipStream_t, const EngineParams&, EngineStats*);
+template s512_status sort_engine<double, false>(double*, int32_t*, size_t, hipStream_t, const EngineParams&, EngineStats*);
+template s512_status sort_engine<int32_t, true>(int32_t*, int32_t*, size_t, hipStream_t, const EngineParams&, EngineStats*);
+
+EngineParams default_params(bool wide) {
+    EngineParams p;
+    p.target_leaf = wide ? 2048 : 4096;
+    p.oversample = 16;
+    p.target_chunks = 1024;
+    p.max_chunk = 1u << 20;
+    p.max_levels = 48;
+    p.seed = 0x5EEDull;
+    return p;
+}
+
+// ---------------------------------------------------------------- host pointers
+// Run `fn` on device memory: in place for device pointers, through a staging
+// copy for host pointers.
+template <class T, class Fn>
+static s512_status with_device_array(T* a, size_t n, Fn fn) {
+    if (!a) {
+        set_error("null array pointer");
+        return S512_ERR_INVALID;
+    }
+    if (is_device_pointer(a)) {
+        return fn(a, (hipStream_t)0);
+    }
+    int dev;
+    s512_status rc = current_device(&dev);
+    if (rc) return rc;
+    T* d = nullptr;
+    S512_HIP(hipMalloc(&d, n * sizeof(T)));
+    hipError_t e = hipMemcpy(d, a, n * sizeof(T), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        rc = fn(d, (hipStream_t)0);
+        if (rc == S512_OK) e = hipMemcpy(a, d, n * sizeof(T), hipMemcpyDeviceToHost);
+    }
+    (void)hipFree(d);
+    if (e != hipSuccess) {
+        set_error("host staging copy failed: %s", hipGetErrorString(e));
+        return S512_ERR_HIP;
+    }
+    return rc;
+}
+
+}  // namespace s512
+
+using namespace s512;
+
+extern "C" s512_status s512_sort_i32_dev(int32_t* d_a, size_t n, void* stream) {
+    if (n > 1 && !d_a) { set_error("null array"); return S512_ERR_INVALID; }
+    return sort_engine<int32_t, false>(d_a, nullptr, n, (hipStream_t)stream, default_params(false), nullptr);
+}
+extern "C" s512_status s512_sort_f64_dev(double* d_a, size_t n, void* stream) {
+    if (n > 1 && !d_a) { set_error("null array"); return S512_ERR_INVALID; }
+    return sort_engine<double, false>(d_a, nullptr, n, (hipStream_t)stream, default_params(true), nullptr);
+}
+extern "C" s512_status s512_sort_i32(int32_t* a, size_t n) {
+    if (n <= 1) return S512_OK;
+    return with_device_array(a, n, [n](int32_t* d, hipStream_t st) { return s512_sort_i32_dev(d, n, st); });
+}
+extern "C" s512_status s512_sort_f64(double* a, size_t n) {
+    if (n <= 1) return S512_OK;
+    return with_device_array(a, n, [n](double* d, hipStream_t st) { return s512_sort_f64_dev(d, n, st); });
+}
+extern "C" s512_status s512kv_sort_i32_dev(int32_t* d_k, int32_t* d_v, size_t n, void* stream) {
+    if (n > 1 && (!d_k || !d_v)) { set_error("null array"); return S512_ERR_INVALID; }
+    return sort_engine<int32_t, true>(d_k, d_v, n, (hipStream_t)stream, default_params(true), nullptr);
+}
+extern "C" s512_status s512kv_sort_i32(int32_t* k, int32_t* v, size_t n) {
+    if (n <= 1) return S512_OK;
+    if (!k || !v) { set_error("null array"); return S512_ERR_INVALID; }
+    return with_device_array(k, n, [&](int32_t* dk, hipStream_t st) {
+        return with_device_array(v, n, [&](int32_t* dv, hipStream_t st2) { return s512kv_sort_i32_dev(dk, dv, n, st2); });
+    });
+}

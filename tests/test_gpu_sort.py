@@ -1,0 +1,145 @@
+"""GPU parity: Sort512::Sort / Sort512kv::Sort through the C ABI vs the oracle.
+
+Reference behaviour pinned (sort512.hpp:5960-5963, sort512kv.hpp:6132-6135):
+the output is the unique ascending permutation of the input; key/value pairs
+stay together.  The oracle's restated quicksort and its closed-form qsort
+both produce it; sizes here are ones the oracle finishes in seconds.
+"""
+import numpy as np
+import pytest
+
+from gpu_util import assert_keys_equal, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [0, 1, 2, 3, 17, 255, 256, 257, 1000, 4096, 8191, 8192, 8193, 16384, 16385,
+         65536, 100_000, 262_145, 1 << 20, 3_000_001]
+
+
+def _inputs(orc, dtype, kind, n, seed):
+    if dtype == np.int32:
+        if kind == "uniform":
+            return orc.gen_i32_full(n, seed)
+        if kind == "small":
+            return (orc.gen_i32_full(n, seed) & 0xFF).astype(np.int32)
+        if kind == "sorted":
+            return np.arange(n, dtype=np.int32) - n // 2
+        if kind == "reversed":
+            return (n // 2 - np.arange(n, dtype=np.int64)).astype(np.int32)
+        if kind == "equal":
+            return np.full(n, 7, dtype=np.int32)
+        if kind == "extremes":
+            a = orc.gen_i32_full(n, seed)
+            a[::3] = np.iinfo(np.int32).max
+            a[1::5] = np.iinfo(np.int32).min
+            return a
+    else:
+        if kind in ("uniform", "normal", "distinct", "sorted", "reversed"):
+            return orc.gen_f64(kind, n, seed)
+        if kind == "equal":
+            return np.full(n, -2.5)
+        if kind == "zeros":
+            a = orc.gen_f64("normal", n, seed)
+            a[::2] = 0.0
+            a[1::4] = -0.0
+            return a
+        if kind == "extremes":
+            a = orc.gen_f64("normal", n, seed) * 1e300
+            a[::7] = np.finfo(np.float64).max
+            a[1::7] = -np.finfo(np.float64).max
+            a[2::11] = np.finfo(np.float64).tiny / 4
+            return a
+    raise ValueError(kind)
+
+
+@pytest.mark.parametrize("n", SIZES)
+@pytest.mark.parametrize("dtype", [np.int32, np.float64])
+def test_sort_uniform_sizes(s512, orc, dev, n, dtype):
+    a = _inputs(orc, dtype, "uniform", n, seed=11 + n)
+    want = orc.closed_sort(a.copy())
+    t = to_dev(a, dev)
+    s512.sort(t)
+    assert_keys_equal(to_host(t), want, f"n={n} {np.dtype(dtype).name}")
+
+
+@pytest.mark.parametrize("kind", ["small", "sorted", "reversed", "equal", "extremes"])
+@pytest.mark.parametrize("n", [300, 9000, 200_000, 2_000_000])
+def test_sort_i32_distributions(s512, orc, dev, kind, n):
+    a = _inputs(orc, np.int32, kind, n, seed=5)
+    want = orc.closed_sort(a.copy())
+    t = to_dev(a, dev)
+    s512.sort(t)
+    assert_keys_equal(to_host(t), want, f"{kind} n={n}")
+
+
+@pytest.mark.parametrize("kind", ["normal", "distinct", "sorted", "reversed", "equal", "zeros", "extremes"])
+@pytest.mark.parametrize("n", [300, 9000, 200_000, 2_000_000])
+def test_sort_f64_distributions(s512, orc, dev, kind, n):
+    a = _inputs(orc, np.float64, kind, n, seed=6)
+    want = orc.closed_sort(a.copy())
+    t = to_dev(a, dev)
+    s512.sort(t)
+    assert_keys_equal(to_host(t), want, f"{kind} n={n}")
+
+
+def test_config1_against_restated_reference(s512, orc, dev):
+    """1,048,576 int32 keys, full int32 range, seed 0: bit-exact vs the
+    oracle's restatement of Sort512::Sort (quicksort + bitonic leaves)."""
+    a = orc.gen_i32_full(1 << 20, 0)
+    want = orc.sort(a.copy())
+    assert np.array_equal(want, np.sort(a))
+    t = to_dev(a, dev)
+    s512.sort(t)
+    assert_keys_equal(to_host(t), want, "config 1")
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.float64])
+def test_sort_host_pointer(s512, orc, dtype):
+    """Plain entry point with host memory: staged through the GPU."""
+    a = _inputs(orc, dtype, "uniform", 123_457, seed=3)
+    want = orc.closed_sort(a.copy())
+    s512.sort(a)
+    assert_keys_equal(a, want, "host pointer")
+
+
+# ---------------------------------------------------------------- key/value
+def _kv_check(orc, k0, v0, k, v, what):
+    want_k = orc.closed_sort(k0.copy())
+    assert_keys_equal(k, want_k, what + " keys")
+    assert np.array_equal(orc.pair_multiset(k, v), orc.pair_multiset(k0, v0)), what + " pairs"
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 100, 256, 257, 5000, 8192, 8193, 70_000, 1 << 20])
+def test_kv_sort_reference_invariant(s512, orc, dev, n):
+    """sort512test.cpp:1409-1427: values = key*100+1 must still match."""
+    orc.srand48(0)
+    k0 = orc.rand_vec(n, np.int32)
+    v0 = (k0.astype(np.int64) * 100 + 1).astype(np.int32)
+    tk, tv = to_dev(k0, dev), to_dev(v0, dev)
+    s512.kv_sort(tk, tv)
+    k, v = to_host(tk), to_host(tv)
+    _kv_check(orc, k0, v0, k, v, f"kv n={n}")
+    assert np.array_equal(v, (k.astype(np.int64) * 100 + 1).astype(np.int32))
+
+
+@pytest.mark.parametrize("kind", ["uniform", "small", "equal", "extremes", "sorted"])
+@pytest.mark.parametrize("n", [300, 20_000, 1_500_000])
+def test_kv_sort_distributions(s512, orc, dev, kind, n):
+    k0 = _inputs(orc, np.int32, kind, n, seed=9)
+    v0 = np.arange(n, dtype=np.int32)
+    tk, tv = to_dev(k0, dev), to_dev(v0, dev)
+    s512.kv_sort(tk, tv)
+    _kv_check(orc, k0, v0, to_host(tk), to_host(tv), f"kv {kind} n={n}")
+
+
+def test_kv_intmax_keys_keep_their_values(s512, orc, dev):
+    """Real INT_MAX keys in short leaves keep their own values (the reference
+    pads leaves with (INT_MAX, INT_MAX) pairs and can emit a pad instead)."""
+    rng = np.random.default_rng(4)
+    for n in (5, 17, 100, 255):
+        k0 = rng.integers(-100, 100, n).astype(np.int32)
+        k0[rng.integers(0, n, max(1, n // 4))] = np.iinfo(np.int32).max
+        v0 = np.arange(n, dtype=np.int32) + 1000
+        tk, tv = to_dev(k0, dev), to_dev(v0, dev)
+        s512.kv_sort(tk, tv)
+        _kv_check(orc, k0, v0, to_host(tk), to_host(tv), f"INT_MAX n={n}")
