@@ -12,17 +12,16 @@ struct EngineParams {
     uint32_t target_chunks;  // chunks per level (load balance vs. histogram size)
     uint32_t max_chunk;      // keys per chunk, upper bound
     int max_levels;          // safety bound on partition levels
+    int max_logf;            // cap on log2 fan-out per level
     uint64_t seed;           // sample positions
 };
 
-struct EngineStats {
-    int levels;
-    uint64_t tiles;
-};
-
 EngineParams default_params(bool wide_keys);
+// Defaults overridden by the non-zero fields of `opt` (may be null).
+EngineParams params_from(const s512_options* opt, bool wide_keys);
 
 template <class T, bool KV>
-s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const EngineParams& prm, EngineStats* stats);
+s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const EngineParams& prm, s512_stats* stats,
+                        bool profile);
 
 }  // namespace s512

@@ -328,7 +328,7 @@ __global__ __launch_bounds__(CNT_P) void k_count(const PartSeg* ps, const ChunkR
 // ---------------------------------------------------------------- column scan
 // hist[chunk][bucket] -> prefix over earlier chunks of the same bucket;
 // bstart[b] = first position of bucket b inside the segment; bsize[b] = size.
-__global__ __launch_bounds__(BMAX) void k_colscan(const PartSeg* ps, uint32_t* hist, uint64_t* bstart, uint64_t* bsize) {
+[[maybe_unused]] static __global__ __launch_bounds__(BMAX) void k_colscan(const PartSeg* ps, uint32_t* hist, uint64_t* bstart, uint64_t* bsize) {
     __shared__ uint32_t wtot[BMAX / 64];
     const PartSeg s = ps[blockIdx.x];
     const uint32_t B = 2u << s.logf;
@@ -448,6 +448,7 @@ __global__ __launch_bounds__(SCAT_P) void k_scatter(const PartSeg* ps, const Chu
 struct PlanLists {
     TileSeg* tile[8];         // per tile class
     uint32_t* tile_count;     // [8]
+    uint64_t* tile_keys;      // [8] keys per class; [8] keys copied back
     uint32_t tile_cap[8];     // class capacities (keys)
     uint32_t nclass;
     TileSeg* copy;            // pieces to copy scratch -> caller's array
@@ -456,36 +457,65 @@ struct PlanLists {
     uint32_t* next_count;
 };
 
-__global__ __launch_bounds__(256) void k_plan(const PartSeg* ps, const uint64_t* bstart, const uint64_t* bsize,
+// Buckets are counted per workgroup in LDS first, so the global list counters
+// see one atomic per (workgroup, list) instead of one per bucket.
+[[maybe_unused]] static __global__ __launch_bounds__(256) void k_plan(const PartSeg* ps, const uint64_t* bstart, const uint64_t* bsize,
                                               PlanLists pl) {
+    constexpr int NL = 10;  // lists: tile classes 0..7, copy = 8, next = 9
+    __shared__ uint32_t lcnt[NL], lbase[NL];
+    __shared__ unsigned long long lkeys[NL];
     const PartSeg s = ps[blockIdx.x];
     const uint32_t B = 2u << s.logf;
     const uint32_t b = blockIdx.y * blockDim.x + threadIdx.x;
-    if (b >= B) return;
-    const uint64_t sz = bsize[s.bkt_base + b];
-    if (sz == 0) return;
-    const uint64_t begin = s.begin + bstart[s.bkt_base + b];
+    if (threadIdx.x < NL) {
+        lcnt[threadIdx.x] = 0;
+        lkeys[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    int list = -1;
+    uint32_t rank = 0, np = 1;
+    uint64_t sz = 0, begin = 0;
     const uint32_t buf = 1u - s.buf;
-    if ((b & 1u) || sz == 1) {
-        if (buf == 1) {
-            const uint32_t np = (uint32_t)((sz + COPY_PIECE - 1) / COPY_PIECE);
-            const uint32_t o = atomicAdd(pl.copy_count, np);
-            for (uint32_t p = 0; p < np; ++p) {
-                const uint64_t pb = (uint64_t)p * COPY_PIECE;
-                pl.copy[o + p] = TileSeg{begin + pb, (uint32_t)((sz - pb) < COPY_PIECE ? (sz - pb) : COPY_PIECE), 1u};
+    if (b < B) {
+        sz = bsize[s.bkt_base + b];
+        begin = s.begin + bstart[s.bkt_base + b];
+        if (sz == 0) {
+            list = -1;
+        } else if ((b & 1u) || sz == 1) {
+            if (buf == 1) {
+                list = 8;
+                np = (uint32_t)((sz + COPY_PIECE - 1) / COPY_PIECE);
             }
+        } else {
+            list = 9;
+            for (uint32_t k = 0; k < pl.nclass; ++k)
+                if (sz <= pl.tile_cap[k]) { list = (int)k; break; }
         }
-        return;
-    }
-    for (uint32_t k = 0; k < pl.nclass; ++k) {
-        if (sz <= pl.tile_cap[k]) {
-            const uint32_t o = atomicAdd(&pl.tile_count[k], 1u);
-            pl.tile[k][o] = TileSeg{begin, (uint32_t)sz, buf};
-            return;
+        if (list >= 0) {
+            rank = atomicAdd(&lcnt[list], np);
+            atomicAdd(&lkeys[list], (unsigned long long)sz);
         }
     }
-    const uint32_t o = atomicAdd(pl.next_count, 1u);
-    pl.next[o] = Seg{begin, sz, buf, 0u};
+    __syncthreads();
+    if (threadIdx.x < NL && lcnt[threadIdx.x]) {
+        const uint32_t l = threadIdx.x;
+        uint32_t* cnt = l < 8 ? &pl.tile_count[l] : (l == 8 ? pl.copy_count : pl.next_count);
+        lbase[l] = atomicAdd(cnt, lcnt[l]);
+        if (l <= 8) atomicAdd((unsigned long long*)&pl.tile_keys[l], lkeys[l]);
+    }
+    __syncthreads();
+    if (list < 0) return;
+    const uint32_t o = lbase[list] + rank;
+    if (list < 8) {
+        pl.tile[list][o] = TileSeg{begin, (uint32_t)sz, buf};
+    } else if (list == 8) {
+        for (uint32_t p = 0; p < np; ++p) {
+            const uint64_t pb = (uint64_t)p * COPY_PIECE;
+            pl.copy[o + p] = TileSeg{begin + pb, (uint32_t)((sz - pb) < COPY_PIECE ? (sz - pb) : COPY_PIECE), 1u};
+        }
+    } else {
+        pl.next[o] = Seg{begin, sz, buf, 0u};
+    }
 }
 
 // ---------------------------------------------------------------- copy

@@ -83,6 +83,47 @@ struct DeviceState {
     DevBuf stage;     // caller-host-pointer staging
     HostBuf pinned;   // host<->device metadata staging
     HostBuf pinned2;
+    std::vector<hipEvent_t> events;  // profiling event pool
+};
+
+// Per-call kernel timing with HIP events recorded on the call's stream.
+struct Profiler {
+    bool on = false;
+    hipStream_t st = nullptr;
+    DeviceState* ds = nullptr;
+    size_t used = 0;
+    std::vector<int> phase;
+    hipEvent_t ev(size_t i) {
+        while (ds->events.size() <= i) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            ds->events.push_back(e);
+        }
+        return ds->events[i];
+    }
+    void begin(int ph) {
+        if (!on) return;
+        hipEvent_t e = ev(used);
+        if (e) (void)hipEventRecord(e, st);
+        used++;
+        phase.push_back(ph);
+    }
+    void end() {
+        if (!on) return;
+        hipEvent_t e = ev(used);
+        if (e) (void)hipEventRecord(e, st);
+        used++;
+    }
+    // after the stream has been synchronised
+    void collect(double* ms, uint32_t* launches) {
+        if (!on) return;
+        for (size_t k = 0; k < phase.size(); ++k) {
+            float t = 0.f;
+            hipEvent_t a = ev(2 * k), b = ev(2 * k + 1);
+            if (a && b && hipEventElapsedTime(&t, a, b) == hipSuccess) ms[phase[k]] += t;
+            launches[phase[k]] += 1;
+        }
+    }
 };
 DeviceState& device_state(int dev);
 s512_status current_device(int* dev);

@@ -91,9 +91,11 @@ static inline int ceil_log2(uint64_t x) {
 
 // ---------------------------------------------------------------- engine
 template <class T, bool KV>
-s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const EngineParams& prm, EngineStats* stats) {
+s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const EngineParams& prm, s512_stats* stats,
+                        bool profile) {
     using K = typename KeyTraits<T>::K;
     using TC = TileCfg<T, KV>;
+    if (stats) memset(stats, 0, sizeof(*stats));
     if (n <= 1) return S512_OK;
     if (n >= (1ull << 32)) {
         set_error("n = %zu exceeds the supported maximum of 2^32-1 keys", n);
@@ -104,6 +106,10 @@ s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const Engin
     if (rc) return rc;
     DeviceState& ds = device_state(dev);
     std::lock_guard<std::mutex> lk(ds.mu);
+    Profiler pf;
+    pf.on = profile && stats;
+    pf.st = st;
+    pf.ds = &ds;
 
     const int ncls = TC::nclass();
     // ---- one tile: no partition level at all
@@ -121,10 +127,16 @@ s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const Engin
         *hseg = TileSeg{0, (uint32_t)n, 0};
         *hcnt = 1;
         S512_HIP(hipMemcpyAsync(ds.arena.ptr, ds.pinned.ptr, sc.off, hipMemcpyHostToDevice, st));
+        pf.begin(S512_PHASE_TILE);
         launch_tile<T, KV>(TC::cls_of(n), dseg, dcnt, 1, a, va, a, va, st);
         S512_CHECK_LAUNCH();
+        pf.end();
         S512_HIP(hipStreamSynchronize(st));  // pinned staging is reused by the next call
-        if (stats) { stats->levels = 0; stats->tiles = 1; }
+        if (stats) {
+            stats->tiles = 1;
+            stats->keys[S512_PHASE_TILE] = n;
+            pf.collect(stats->kernel_ms, stats->launches);
+        }
         return S512_OK;
     }
 
@@ -140,6 +152,7 @@ s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const Engin
     const uint32_t smax = 8192;
     int level = 0;
     uint64_t tiles = 0;
+    uint64_t keys_ph[S512_NPHASE] = {0};
     while (!segs.empty()) {
         if (++level > prm.max_levels) {
             set_error("partition did not converge within %d levels", prm.max_levels);
@@ -160,7 +173,7 @@ s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const Engin
             p.size = segs[i].size;
             p.buf = segs[i].buf;
             int lf = ceil_log2((p.size + prm.target_leaf - 1) / prm.target_leaf);
-            lf = std::max(1, std::min(lf, FMAX_LOG));
+            lf = std::max(1, std::min(lf, prm.max_logf));
             p.logf = (uint32_t)lf;
             const uint32_t F = 1u << lf, B = 2 * F;
             p.nsample = std::min<uint32_t>(smax, prm.oversample * F);
@@ -176,12 +189,13 @@ s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const Engin
         }
         const uint32_t tc = (uint32_t)crs.size();
         const uint64_t copy_cap = sumB + total / COPY_PIECE + 1;
-        // ---- arena layout: uploaded prefix first
+        // ---- arena layout: the uploaded prefix first
         SizeCounter sc;
         sc.take<PartSeg>(m);
         sc.take<ChunkRef>(tc);
         sc.take<TileSeg>(m);
         sc.take<uint32_t>(32);
+        sc.take<uint64_t>(16);
         const size_t up_bytes = sc.off;
         sc.take<K>((size_t)m * smax);
         sc.take<K>(sumB);
@@ -200,6 +214,7 @@ s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const Engin
         ChunkRef* d_cr = ar.take<ChunkRef>(tc);
         TileSeg* d_sl = ar.take<TileSeg>(m);
         uint32_t* d_cnt = ar.take<uint32_t>(32);
+        uint64_t* d_cnt64 = ar.take<uint64_t>(16);
         K* d_smp = ar.take<K>((size_t)m * smax);
         K* d_tree = ar.take<K>(sumB);
         uint32_t* d_hist = ar.take<uint32_t>(histN);
@@ -213,6 +228,7 @@ s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const Engin
         }
         pl.nclass = (uint32_t)ncls;
         pl.tile_count = d_cnt;
+        pl.tile_keys = d_cnt64;
         pl.copy = ar.take<TileSeg>(copy_cap);
         pl.copy_count = d_cnt + 8;
         pl.next = ar.take<Seg>(sumB);
@@ -222,43 +238,69 @@ s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const Engin
         ChunkRef* h_cr = hp.take<ChunkRef>(tc);
         TileSeg* h_sl = hp.take<TileSeg>(m);
         uint32_t* h_cnt = hp.take<uint32_t>(32);
+        uint64_t* h_cnt64 = hp.take<uint64_t>(16);
         memcpy(h_ps, ps.data(), m * sizeof(PartSeg));
         memcpy(h_cr, crs.data(), tc * sizeof(ChunkRef));
         for (uint32_t i = 0; i < m; ++i) h_sl[i] = TileSeg{(uint64_t)i * smax, ps[i].nsample, 0};
         memset(h_cnt, 0, 32 * sizeof(uint32_t));
+        memset(h_cnt64, 0, 16 * sizeof(uint64_t));
         h_cnt[10] = m;
         S512_HIP(hipMemcpyAsync(ds.arena.ptr, ds.pinned.ptr, up_bytes, hipMemcpyHostToDevice, st));
 
         const uint64_t seed = prm.seed + 0x9E37ull * (uint64_t)level;
+        pf.begin(S512_PHASE_SAMPLE);
         k_sample<T><<<m, 256, 0, st>>>(d_ps, a, b, d_smp, smax, seed);
         S512_CHECK_LAUNCH();
+        pf.end();
+        pf.begin(S512_PHASE_SAMPLE_SORT);
         k_tile_sort<K, 512, 16><<<m, 512, 0, st>>>(d_sl, d_cnt + 10, d_smp, d_smp);
         S512_CHECK_LAUNCH();
+        pf.end();
+        pf.begin(S512_PHASE_TREE);
         k_build_tree<K><<<m, FMAX, 0, st>>>(d_ps, d_smp, smax, d_tree);
         S512_CHECK_LAUNCH();
+        pf.end();
+        pf.begin(S512_PHASE_COUNT);
         k_count<T><<<tc, CNT_P, 0, st>>>(d_ps, d_cr, a, b, d_tree, d_hist);
         S512_CHECK_LAUNCH();
+        pf.end();
+        pf.begin(S512_PHASE_SCAN);
         k_colscan<<<m, BMAX, 0, st>>>(d_ps, d_hist, d_bstart, d_bsize);
         S512_CHECK_LAUNCH();
+        pf.end();
+        pf.begin(S512_PHASE_SCATTER);
         k_scatter<T, KV><<<tc, SCAT_P, 0, st>>>(d_ps, d_cr, a, b, va, vb, d_tree, d_hist, d_bstart);
         S512_CHECK_LAUNCH();
+        pf.end();
+        pf.begin(S512_PHASE_PLAN);
         k_plan<<<dim3(m, (uint32_t)((maxB + 255) / 256)), 256, 0, st>>>(d_ps, d_bstart, d_bsize, pl);
         S512_CHECK_LAUNCH();
+        pf.end();
+        keys_ph[S512_PHASE_COUNT] += total;
+        keys_ph[S512_PHASE_SCATTER] += total;
         uint32_t* h_out = (uint32_t*)ds.pinned2.ptr;
-        S512_HIP(hipMemcpyAsync(h_out, d_cnt, 32 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        S512_HIP(hipMemcpyAsync(h_out, d_cnt, 32 * sizeof(uint32_t) + 256, hipMemcpyDeviceToHost, st));
         S512_HIP(hipStreamSynchronize(st));
         uint32_t counts[32];
+        uint64_t counts64[16];
         memcpy(counts, h_out, sizeof(counts));
+        memcpy(counts64, (char*)h_out + ((char*)d_cnt64 - (char*)d_cnt), sizeof(counts64));
         if (counts[8]) {
+            pf.begin(S512_PHASE_COPY);
             k_copy_back<T, KV><<<std::min<uint32_t>(counts[8], 16384), 256, 0, st>>>(pl.copy, pl.copy_count, a, b, va,
                                                                                      vb);
             S512_CHECK_LAUNCH();
+            pf.end();
+            keys_ph[S512_PHASE_COPY] += counts64[8];
         }
         for (int c = 0; c < ncls; ++c) {
             if (!counts[c]) continue;
+            pf.begin(S512_PHASE_TILE);
             launch_tile<T, KV>(c, pl.tile[c], d_cnt + c, counts[c], a, va, b, vb, st);
             S512_CHECK_LAUNCH();
+            pf.end();
             tiles += counts[c];
+            keys_ph[S512_PHASE_TILE] += counts64[c];
         }
         const uint32_t nn = counts[9];
         segs.clear();
@@ -274,13 +316,18 @@ s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const Engin
         }
     }
     S512_HIP(hipStreamSynchronize(st));
-    if (stats) { stats->levels = level; stats->tiles = tiles; }
+    if (stats) {
+        stats->levels = level;
+        stats->tiles = tiles;
+        memcpy(stats->keys, keys_ph, sizeof(keys_ph));
+        pf.collect(stats->kernel_ms, stats->launches);
+    }
     return S512_OK;
 }
 
-template s512_status sort_engine<int32_t, false>(int32_t*, int32_t*, size_t, hipStream_t, const EngineParams&, EngineStats*);
-template s512_status sort_engine<double, false>(double*, int32_t*, size_t, hipStream_t, const EngineParams&, EngineStats*);
-template s512_status sort_engine<int32_t, true>(int32_t*, int32_t*, size_t, hipStream_t, const EngineParams&, EngineStats*);
+template s512_status sort_engine<int32_t, false>(int32_t*, int32_t*, size_t, hipStream_t, const EngineParams&, s512_stats*, bool);
+template s512_status sort_engine<double, false>(double*, int32_t*, size_t, hipStream_t, const EngineParams&, s512_stats*, bool);
+template s512_status sort_engine<int32_t, true>(int32_t*, int32_t*, size_t, hipStream_t, const EngineParams&, s512_stats*, bool);
 
 EngineParams default_params(bool wide) {
     EngineParams p;
@@ -289,7 +336,18 @@ EngineParams default_params(bool wide) {
     p.target_chunks = 1024;
     p.max_chunk = 1u << 20;
     p.max_levels = 48;
+    p.max_logf = FMAX_LOG;
     p.seed = 0x5EEDull;
+    return p;
+}
+
+EngineParams params_from(const s512_options* opt, bool wide) {
+    EngineParams p = default_params(wide);
+    if (!opt) return p;
+    if (opt->target_leaf) p.target_leaf = opt->target_leaf;
+    if (opt->oversample) p.oversample = opt->oversample;
+    if (opt->max_fanout_log2) p.max_logf = std::max(1, std::min<int>((int)opt->max_fanout_log2, FMAX_LOG));
+    if (opt->seed) p.seed = opt->seed;
     return p;
 }
 
@@ -327,13 +385,44 @@ static s512_status with_device_array(T* a, size_t n, Fn fn) {
 
 using namespace s512;
 
-extern "C" s512_status s512_sort_i32_dev(int32_t* d_a, size_t n, void* stream) {
+static s512_status check_opts(const s512_options* opt) {
+    if (opt && opt->oversample > 512) {
+        set_error("oversample %u too large (max 512)", opt->oversample);
+        return S512_ERR_INVALID;
+    }
+    if (opt && opt->target_leaf && opt->target_leaf < 64) {
+        set_error("target_leaf %u too small (min 64)", opt->target_leaf);
+        return S512_ERR_INVALID;
+    }
+    return S512_OK;
+}
+
+extern "C" s512_status s512_sort_i32_dev_ex(int32_t* d_a, size_t n, const s512_options* opt, s512_stats* stats,
+                                            void* stream) {
     if (n > 1 && !d_a) { set_error("null array"); return S512_ERR_INVALID; }
-    return sort_engine<int32_t, false>(d_a, nullptr, n, (hipStream_t)stream, default_params(false), nullptr);
+    if (s512_status rc = check_opts(opt)) return rc;
+    return sort_engine<int32_t, false>(d_a, nullptr, n, (hipStream_t)stream, params_from(opt, false), stats,
+                                       opt && opt->profile);
+}
+extern "C" s512_status s512_sort_f64_dev_ex(double* d_a, size_t n, const s512_options* opt, s512_stats* stats,
+                                            void* stream) {
+    if (n > 1 && !d_a) { set_error("null array"); return S512_ERR_INVALID; }
+    if (s512_status rc = check_opts(opt)) return rc;
+    return sort_engine<double, false>(d_a, nullptr, n, (hipStream_t)stream, params_from(opt, true), stats,
+                                      opt && opt->profile);
+}
+extern "C" s512_status s512kv_sort_i32_dev_ex(int32_t* d_k, int32_t* d_v, size_t n, const s512_options* opt,
+                                              s512_stats* stats, void* stream) {
+    if (n > 1 && (!d_k || !d_v)) { set_error("null array"); return S512_ERR_INVALID; }
+    if (s512_status rc = check_opts(opt)) return rc;
+    return sort_engine<int32_t, true>(d_k, d_v, n, (hipStream_t)stream, params_from(opt, true), stats,
+                                      opt && opt->profile);
+}
+extern "C" s512_status s512_sort_i32_dev(int32_t* d_a, size_t n, void* stream) {
+    return s512_sort_i32_dev_ex(d_a, n, nullptr, nullptr, stream);
 }
 extern "C" s512_status s512_sort_f64_dev(double* d_a, size_t n, void* stream) {
-    if (n > 1 && !d_a) { set_error("null array"); return S512_ERR_INVALID; }
-    return sort_engine<double, false>(d_a, nullptr, n, (hipStream_t)stream, default_params(true), nullptr);
+    return s512_sort_f64_dev_ex(d_a, n, nullptr, nullptr, stream);
 }
 extern "C" s512_status s512_sort_i32(int32_t* a, size_t n) {
     if (n <= 1) return S512_OK;
@@ -344,8 +433,7 @@ extern "C" s512_status s512_sort_f64(double* a, size_t n) {
     return with_device_array(a, n, [n](double* d, hipStream_t st) { return s512_sort_f64_dev(d, n, st); });
 }
 extern "C" s512_status s512kv_sort_i32_dev(int32_t* d_k, int32_t* d_v, size_t n, void* stream) {
-    if (n > 1 && (!d_k || !d_v)) { set_error("null array"); return S512_ERR_INVALID; }
-    return sort_engine<int32_t, true>(d_k, d_v, n, (hipStream_t)stream, default_params(true), nullptr);
+    return s512kv_sort_i32_dev_ex(d_k, d_v, n, nullptr, nullptr, stream);
 }
 extern "C" s512_status s512kv_sort_i32(int32_t* k, int32_t* v, size_t n) {
     if (n <= 1) return S512_OK;
@@ -353,4 +441,20 @@ extern "C" s512_status s512kv_sort_i32(int32_t* k, int32_t* v, size_t n) {
     return with_device_array(k, n, [&](int32_t* dk, hipStream_t st) {
         return with_device_array(v, n, [&](int32_t* dv, hipStream_t st2) { return s512kv_sort_i32_dev(dk, dv, n, st2); });
     });
+}
+
+// SmallSort16V (sort512.hpp:2255 double <= 128 keys, 5312 int <= 256 keys;
+// sort512kv.hpp:5235 pairs <= 256): one wave sorts the array in registers.
+// Lengths above the reference's limit are rejected (undefined there).
+extern "C" s512_status s512_smallsort16v_i32(int32_t* a, size_t len) {
+    if (len > 256) { set_error("SmallSort16V(int) takes at most 256 keys, got %zu", len); return S512_ERR_INVALID; }
+    return s512_sort_i32(a, len);
+}
+extern "C" s512_status s512_smallsort16v_f64(double* a, size_t len) {
+    if (len > 128) { set_error("SmallSort16V(double) takes at most 128 keys, got %zu", len); return S512_ERR_INVALID; }
+    return s512_sort_f64(a, len);
+}
+extern "C" s512_status s512kv_smallsort16v_i32(int32_t* k, int32_t* v, size_t len) {
+    if (len > 256) { set_error("SmallSort16V(int,int) takes at most 256 pairs, got %zu", len); return S512_ERR_INVALID; }
+    return s512kv_sort_i32(k, v, len);
 }

@@ -1,0 +1,81 @@
+// s512_synth.hip -- synthetic inputs generated on the device (benchmark and
+// test utility; no counterpart in the reference).  Same definitions as the
+// oracle's generators (SplitMix64 counter hash of (seed, index)); Box-Muller
+// uses the device's log/sqrt/cos, so normal keys may differ from the host
+// restatement in the last ulp -- parity checks always copy the exact device
+// input to the host.
+#include "s512_common.hpp"
+#include "s512_runtime.hpp"
+#include "../../include/s512_synth.h"
+
+namespace s512 {
+
+__device__ __forceinline__ uint64_t sm64(uint64_t seed, uint64_t i) {
+    uint64_t z = seed * 0x9E3779B97F4A7C15ull + (i + 1) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ double u01(uint64_t seed, uint64_t i) { return (double)(sm64(seed, i) >> 11) * 0x1.0p-53; }
+
+__global__ void k_fill_f64(double* out, uint64_t n, int kind, uint64_t seed, uint64_t param) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        double v;
+        switch (kind) {
+            case S512X_UNIFORM: v = u01(seed, i); break;
+            case S512X_NORMAL: {
+                double u1 = u01(seed, 2 * i), u2 = u01(seed, 2 * i + 1);
+                if (u1 < 0x1.0p-53) u1 = 0x1.0p-53;
+                v = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+                break;
+            }
+            case S512X_DISTINCT: v = (double)(sm64(seed, i) % (param ? param : 1)); break;
+            case S512X_SORTED: v = (double)i / (double)n; break;
+            case S512X_REVERSED: v = (double)(n - 1 - i) / (double)n; break;
+            default: v = 0.0; break;
+        }
+        out[i] = v;
+    }
+}
+
+__global__ void k_fill_i32(int32_t* out, uint64_t n, int kind, uint64_t seed, uint64_t param) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        int32_t v;
+        switch (kind) {
+            case S512X_UNIFORM: v = (int32_t)(uint32_t)(sm64(seed, i) >> 32); break;
+            case S512X_DISTINCT: v = (int32_t)(sm64(seed, i) % (param ? param : 1)); break;
+            case S512X_SORTED: v = (int32_t)(i - n / 2); break;
+            case S512X_REVERSED: v = (int32_t)(n / 2 - i); break;
+            case S512X_INDEX: v = (int32_t)i; break;
+            default: v = 0; break;
+        }
+        out[i] = v;
+    }
+}
+
+}  // namespace s512
+
+using namespace s512;
+
+static unsigned grid_for(size_t n) {
+    size_t g = (n + 255) / 256;
+    return (unsigned)(g < 65536 ? (g ? g : 1) : 65536);
+}
+
+extern "C" s512_status s512x_fill_f64(double* d, size_t n, int kind, uint64_t seed, uint64_t param, void* stream) {
+    if (!d && n) { set_error("null array"); return S512_ERR_INVALID; }
+    if (kind < S512X_UNIFORM || kind > S512X_REVERSED) { set_error("unknown f64 distribution %d", kind); return S512_ERR_INVALID; }
+    if (!n) return S512_OK;
+    k_fill_f64<<<grid_for(n), 256, 0, (hipStream_t)stream>>>(d, n, kind, seed, param);
+    S512_CHECK_LAUNCH();
+    return S512_OK;
+}
+
+extern "C" s512_status s512x_fill_i32(int32_t* d, size_t n, int kind, uint64_t seed, uint64_t param, void* stream) {
+    if (!d && n) { set_error("null array"); return S512_ERR_INVALID; }
+    if (kind == S512X_NORMAL || kind < S512X_UNIFORM || kind > S512X_INDEX) { set_error("unknown i32 distribution %d", kind); return S512_ERR_INVALID; }
+    if (!n) return S512_OK;
+    k_fill_i32<<<grid_for(n), 256, 0, (hipStream_t)stream>>>(d, n, kind, seed, param);
+    S512_CHECK_LAUNCH();
+    return S512_OK;
+}

@@ -23,8 +23,9 @@ namespace s512 {
 // ---------------------------------------------------------------- compare
 template <class K>
 __device__ __forceinline__ void cex_dir(K& a, K& b, bool desc) {
-    // afterwards a <= b (ascending pair) or a >= b (descending pair)
-    const bool sw = desc ? (a < b) : (b < a);
+    // afterwards a <= b (ascending pair) or a >= b (descending pair); for
+    // a == b swapping or not gives the same pair, so one compare suffices
+    const bool sw = (b < a) != desc;
     const K x = sw ? b : a;
     const K y = sw ? a : b;
     a = x;
@@ -137,33 +138,76 @@ struct BlockBitonic {
     // XOR swizzle so that the blocked layout does not put a wave on one bank.
     __device__ static __forceinline__ uint32_t phys(uint32_t i) { return i ^ ((i >> LOGE) & (E - 1)); }
 
-    __device__ static __forceinline__ void relayout(K (&v)[E], K* lds, int wf, int wt, uint32_t t) {
+    template <int WF, int WT>
+    __device__ static __forceinline__ void relayout(K (&v)[E], K* lds, uint32_t t) {
 #pragma unroll
-        for (int r = 0; r < E; ++r) lds[phys(idx(t, r, wf))] = v[r];
+        for (int r = 0; r < E; ++r) lds[phys(idx(t, r, WF))] = v[r];
         __syncthreads();
 #pragma unroll
-        for (int r = 0; r < E; ++r) v[r] = lds[phys(idx(t, r, wt))];
+        for (int r = 0; r < E; ++r) v[r] = lds[phys(idx(t, r, WT))];
         __syncthreads();
+    }
+
+    // Stages on index bits B, B-1, .., W of merge size 2^Q while the window
+    // [W, W+LOGE) is in the registers.  The direction of a pair is bit Q of
+    // its index: a register bit when Q lies inside the window, otherwise bit
+    // Q-LOGE of the thread id (or ascending for the final merge Q == LOGN).
+    template <int Q, int W, int B>
+    __device__ static __forceinline__ void win_stages(K (&v)[E], uint32_t t) {
+        if constexpr (B >= W) {
+            constexpr int PB = B - W;
+            if constexpr (Q < W + LOGE) {
+#pragma unroll
+                for (int r = 0; r < E; ++r) {
+                    if (r & (1 << PB)) continue;
+                    if ((r >> (Q - W)) & 1) cex_up(v[r | (1 << PB)], v[r]);
+                    else cex_up(v[r], v[r | (1 << PB)]);
+                }
+            } else if constexpr (Q == LOGN) {
+#pragma unroll
+                for (int r = 0; r < E; ++r) {
+                    if (r & (1 << PB)) continue;
+                    cex_up(v[r], v[r | (1 << PB)]);
+                }
+            } else {
+                const bool desc = ((t >> (Q - LOGE)) & 1u) != 0;
+#pragma unroll
+                for (int r = 0; r < E; ++r) {
+                    if (r & (1 << PB)) continue;
+                    cex_dir(v[r], v[r | (1 << PB)], desc);
+                }
+            }
+            win_stages<Q, W, B - 1>(v, t);
+        }
+    }
+
+    // All stages of merge size 2^Q above the register bits, window by window
+    // from the top bit down, then back to the blocked layout for the rest.
+    template <int Q, int HI, int WPREV>
+    __device__ static __forceinline__ void windows(K (&v)[E], K* lds, uint32_t t) {
+        if constexpr (HI >= LOGE) {
+            constexpr int NW = (HI - LOGE + 1) > LOGE ? (HI - LOGE + 1) : LOGE;
+            relayout<WPREV, NW>(v, lds, t);
+            win_stages<Q, NW, HI>(v, t);
+            windows<Q, NW - 1, NW>(v, lds, t);
+        } else {
+            relayout<WPREV, 0>(v, lds, t);
+            win_stages<Q, 0, LOGE - 1>(v, t);
+        }
+    }
+
+    template <int Q>
+    __device__ static __forceinline__ void merges(K (&v)[E], K* lds, uint32_t t) {
+        if constexpr (Q <= LOGN) {
+            windows<Q, Q - 1, 0>(v, lds, t);
+            merges<Q + 1>(v, lds, t);
+        }
     }
 
     // v holds the blocked layout on entry and on exit; sorted ascending.
     __device__ static void sort(K (&v)[E], K* lds, uint32_t t) {
         reg_sort<K, E>(v, t);
-        for (int q = LOGE + 1; q <= LOGN; ++q) {
-            int w = 0;
-            int hi = q - 1;
-            while (hi >= LOGE) {
-                const int nw = (hi - LOGE + 1) > LOGE ? (hi - LOGE + 1) : LOGE;
-                relayout(v, lds, w, nw, t);
-                w = nw;
-                const uint32_t i0 = idx(t, 0, w);
-                for (int b = hi; b >= w; --b) reg_stage_any<K, E>(v, b - w, i0, w, q, LOGN);
-                hi = w - 1;
-            }
-            relayout(v, lds, w, 0, t);
-            const bool desc = (q < LOGN) && (((t >> (q - LOGE)) & 1u) != 0);
-            reg_low_stages<K, E>(v, desc);
-        }
+        merges<LOGE + 1>(v, lds, t);
     }
 
     // striped global -> blocked registers, through LDS

@@ -51,11 +51,87 @@ _SIGS = {
     "s512_sort_f64_dev": ([_vp, _c_size, _vp], ctypes.c_int),
     "s512kv_sort_i32": ([_vp, _vp, _c_size], ctypes.c_int),
     "s512kv_sort_i32_dev": ([_vp, _vp, _c_size, _vp], ctypes.c_int),
+    "s512_sort_i32_dev_ex": ([_vp, _c_size, _vp, _vp, _vp], ctypes.c_int),
+    "s512_sort_f64_dev_ex": ([_vp, _c_size, _vp, _vp, _vp], ctypes.c_int),
+    "s512kv_sort_i32_dev_ex": ([_vp, _vp, _c_size, _vp, _vp, _vp], ctypes.c_int),
+    "s512_sort_merge_i32": ([_vp, _c_size], ctypes.c_int),
+    "s512_sort_merge_f64": ([_vp, _c_size], ctypes.c_int),
+    "s512_sort_merge_i32_dev": ([_vp, _c_size, _vp], ctypes.c_int),
+    "s512_sort_merge_f64_dev": ([_vp, _c_size, _vp], ctypes.c_int),
+    "s512_partition_i32": ([_vp, _c_size, _c_size, ctypes.c_int32, ctypes.POINTER(_c_size)], ctypes.c_int),
+    "s512_partition_f64": ([_vp, _c_size, _c_size, ctypes.c_double, ctypes.POINTER(_c_size)], ctypes.c_int),
+    "s512_partition_i32_dev": ([_vp, _c_size, _c_size, ctypes.c_int32, ctypes.POINTER(_c_size), _vp], ctypes.c_int),
+    "s512_partition_f64_dev": ([_vp, _c_size, _c_size, ctypes.c_double, ctypes.POINTER(_c_size), _vp], ctypes.c_int),
+    "s512kv_partition_i32": ([_vp, _vp, _c_size, _c_size, ctypes.c_int32, ctypes.POINTER(_c_size)], ctypes.c_int),
+    "s512kv_partition_i32_dev": ([_vp, _vp, _c_size, _c_size, ctypes.c_int32, ctypes.POINTER(_c_size), _vp],
+                                 ctypes.c_int),
+    "s512_smallsort16v_i32": ([_vp, _c_size], ctypes.c_int),
+    "s512_smallsort16v_f64": ([_vp, _c_size], ctypes.c_int),
+    "s512kv_smallsort16v_i32": ([_vp, _vp, _c_size], ctypes.c_int),
+    "s512_merge_inplace_i32": ([_vp, _c_size, _c_size], ctypes.c_int),
+    "s512_merge_inplace_f64": ([_vp, _c_size, _c_size], ctypes.c_int),
+    "s512_merge_inplace_i32_dev": ([_vp, _c_size, _c_size, _vp], ctypes.c_int),
+    "s512_merge_inplace_f64_dev": ([_vp, _c_size, _c_size, _vp], ctypes.c_int),
+    "s512x_fill_f64": ([_vp, _c_size, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, _vp], ctypes.c_int),
+    "s512x_fill_i32": ([_vp, _c_size, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, _vp], ctypes.c_int),
 }
 for _name, (_args, _res) in _SIGS.items():
     _f = getattr(lib, _name)
     _f.argtypes = _args
     _f.restype = _res
+
+
+NPHASE = 10
+PHASES = ["sample", "sample_sort", "tree", "count", "scan", "scatter", "plan", "copy", "tile", "train"]
+
+
+class Options(ctypes.Structure):
+    """s512_options (include/s512.h)."""
+    _fields_ = [("target_leaf", ctypes.c_uint32), ("oversample", ctypes.c_uint32),
+                ("max_fanout_log2", ctypes.c_uint32), ("profile", ctypes.c_uint32), ("seed", ctypes.c_uint64)]
+
+
+class Stats(ctypes.Structure):
+    """s512_stats (include/s512.h)."""
+    _fields_ = [("levels", ctypes.c_int32), ("tiles", ctypes.c_uint64), ("kernel_ms", ctypes.c_double * NPHASE),
+                ("launches", ctypes.c_uint32 * NPHASE), ("keys", ctypes.c_uint64 * NPHASE)]
+
+    def as_dict(self):
+        return {"levels": self.levels, "tiles": self.tiles,
+                "phases": {PHASES[i]: {"ms": self.kernel_ms[i], "launches": self.launches[i], "keys": self.keys[i]}
+                           for i in range(NPHASE) if self.launches[i] or self.keys[i]}}
+
+
+SYNTH = {"uniform": 0, "normal": 1, "distinct": 2, "sorted": 3, "reversed": 4, "index": 5}
+
+
+def fill(t, kind, seed=0, param=0):
+    """Device-side synthetic input (include/s512_synth.h) into a torch tensor."""
+    k = _kind(t)
+    _check(getattr(lib, f"s512x_fill_{k}")(_ptr(t), t.numel(), SYNTH[kind], seed, param, _stream_of(t)))
+    return t
+
+
+def sort_ex(a, options=None, profile=False):
+    """Sort with options and statistics (device tensors); returns Stats."""
+    k = _kind(a)
+    opt = options or Options()
+    if profile:
+        opt.profile = 1
+    st = Stats()
+    _check(getattr(lib, f"s512_sort_{k}_dev_ex")(_ptr(a), a.numel(), ctypes.byref(opt), ctypes.byref(st),
+                                                 _stream_of(a)))
+    return st
+
+
+def kv_sort_ex(keys, values, options=None, profile=False):
+    opt = options or Options()
+    if profile:
+        opt.profile = 1
+    st = Stats()
+    _check(lib.s512kv_sort_i32_dev_ex(_ptr(keys), _ptr(values), keys.numel(), ctypes.byref(opt), ctypes.byref(st),
+                                      _stream_of(keys)))
+    return st
 
 
 def declared_symbols():
@@ -131,3 +207,67 @@ def kv_sort(keys, values):
     else:
         _check(lib.s512kv_sort_i32(_ptr(keys), _ptr(values), n))
     return keys, values
+
+
+def _n(x):
+    return x.numel() if _is_torch(x) else x.size
+
+
+def _dev(x):
+    return _is_torch(x) and x.is_cuda
+
+
+def sort_merge(a):
+    """Sort512::SortOmpMerge / SortOmpMergeDeps / SortOmpParMerge: GPU merge sort."""
+    k = _kind(a)
+    if _dev(a):
+        _check(getattr(lib, f"s512_sort_merge_{k}_dev")(_ptr(a), _n(a), _stream_of(a)))
+    else:
+        _check(getattr(lib, f"s512_sort_merge_{k}")(_ptr(a), _n(a)))
+    return a
+
+
+def partition(a, left, right, pivot):
+    """Sort512::Partition512: returns left + #{a[left..right] <= pivot}."""
+    k = _kind(a)
+    out = _c_size()
+    cast = int if k == "i32" else float
+    if _dev(a):
+        _check(getattr(lib, f"s512_partition_{k}_dev")(_ptr(a), left, right, cast(pivot), ctypes.byref(out),
+                                                       _stream_of(a)))
+    else:
+        _check(getattr(lib, f"s512_partition_{k}")(_ptr(a), left, right, cast(pivot), ctypes.byref(out)))
+    return out.value
+
+
+def kv_partition(keys, values, left, right, pivot):
+    """Sort512kv::Partition512."""
+    out = _c_size()
+    if _dev(keys):
+        _check(lib.s512kv_partition_i32_dev(_ptr(keys), _ptr(values), left, right, int(pivot), ctypes.byref(out),
+                                            _stream_of(keys)))
+    else:
+        _check(lib.s512kv_partition_i32(_ptr(keys), _ptr(values), left, right, int(pivot), ctypes.byref(out)))
+    return out.value
+
+
+def smallsort16v(a):
+    """Sort512::SmallSort16V (<= 256 int32 / <= 128 float64 keys)."""
+    _check(getattr(lib, f"s512_smallsort16v_{_kind(a)}")(_ptr(a), _n(a)))
+    return a
+
+
+def kv_smallsort16v(keys, values):
+    """Sort512kv::SmallSort16V (<= 256 pairs)."""
+    _check(lib.s512kv_smallsort16v_i32(_ptr(keys), _ptr(values), _n(keys)))
+    return keys, values
+
+
+def merge_inplace(a, center):
+    """ParallelInplace::parallelMergeInPlace: merge sorted a[:center] and a[center:]."""
+    k = _kind(a)
+    if _dev(a):
+        _check(getattr(lib, f"s512_merge_inplace_{k}_dev")(_ptr(a), _n(a), center, _stream_of(a)))
+    else:
+        _check(getattr(lib, f"s512_merge_inplace_{k}")(_ptr(a), _n(a), center))
+    return a

@@ -60,12 +60,98 @@ s512_status s512_sort_f64(double* a, size_t n);
 s512_status s512_sort_i32_dev(int32_t* d_a, size_t n, void* stream);
 s512_status s512_sort_f64_dev(double* d_a, size_t n, void* stream);
 
+/* SortOmp / Sort512::SortOmpPartition<T,size_t>  sort512.hpp:5992-6004 (the
+ * README's "SortOmp", sort512.hpp:11, which the header does not define) map to
+ * the same GPU sort as Sort512::Sort: s512_sort_i32 / s512_sort_f64.        */
+
+/* Sort512::SortOmpMerge / SortOmpMergeDeps / SortOmpParMerge
+ * sort512.hpp:6006-6058, 6060-6102, 6104-6174: sort chunks, then merge runs
+ * pairwise.  GPU merge sort: bitonic tiles + merge-path rounds.             */
+s512_status s512_sort_merge_i32(int32_t* a, size_t n);
+s512_status s512_sort_merge_f64(double* a, size_t n);
+s512_status s512_sort_merge_i32_dev(int32_t* d_a, size_t n, void* stream);
+s512_status s512_sort_merge_f64_dev(double* d_a, size_t n, void* stream);
+
+/* ------------------------------------------------------------- partition */
+/* Sort512::Partition512<size_t>(int[], left, right, pivot)    sort512.hpp:5729
+ * Sort512::Partition512<size_t>(double[], left, right, pivot) sort512.hpp:5821
+ * Rearranges the inclusive range [left, right] of a (indices from a) so that
+ * keys <= pivot come first; *out_bound = left + #{keys <= pivot}.  Both sides
+ * keep their input order.  right = left - 1 is an empty range.             */
+s512_status s512_partition_i32(int32_t* a, size_t left, size_t right, int32_t pivot, size_t* out_bound);
+s512_status s512_partition_f64(double* a, size_t left, size_t right, double pivot, size_t* out_bound);
+s512_status s512_partition_i32_dev(int32_t* d_a, size_t left, size_t right, int32_t pivot, size_t* out_bound,
+                                   void* stream);
+s512_status s512_partition_f64_dev(double* d_a, size_t left, size_t right, double pivot, size_t* out_bound,
+                                   void* stream);
+
+/* ------------------------------------------------------------ small sort */
+/* Sort512::SmallSort16V(double*, len <= 128)  sort512.hpp:2255
+ * Sort512::SmallSort16V(int*, len <= 256)     sort512.hpp:5312
+ * One wave64 bitonic network.  Longer lengths return S512_ERR_INVALID.     */
+s512_status s512_smallsort16v_i32(int32_t* a, size_t len);
+s512_status s512_smallsort16v_f64(double* a, size_t len);
+
+/* ----------------------------------------------------------------- merge */
+/* ParallelInplace::parallelMergeInPlace(array, sizeArray, centerPosition, ...)
+ * parallelInplace.hpp:209-306: merge the sorted runs [0, center) and
+ * [center, size) of a in place.  Stable (ties keep the left run first).    */
+s512_status s512_merge_inplace_i32(int32_t* a, size_t size, size_t center);
+s512_status s512_merge_inplace_f64(double* a, size_t size, size_t center);
+s512_status s512_merge_inplace_i32_dev(int32_t* d_a, size_t size, size_t center, void* stream);
+s512_status s512_merge_inplace_f64_dev(double* d_a, size_t size, size_t center, void* stream);
+
+/* ------------------------------------------------------ options and stats */
+/* Tuning knobs and per-call observability.  Zero fields select defaults.  */
+enum {
+    S512_PHASE_SAMPLE = 0,      /* draw splitter samples                    */
+    S512_PHASE_SAMPLE_SORT = 1, /* bitonic sort of the samples              */
+    S512_PHASE_TREE = 2,        /* splitter tree / learned model tabulation */
+    S512_PHASE_COUNT = 3,       /* classify + per-chunk bucket histogram    */
+    S512_PHASE_SCAN = 4,        /* bucket offsets                           */
+    S512_PHASE_SCATTER = 5,     /* classify + LDS-staged scatter            */
+    S512_PHASE_PLAN = 6,        /* route buckets                            */
+    S512_PHASE_COPY = 7,        /* finished runs back to the caller array   */
+    S512_PHASE_TILE = 8,        /* bitonic tile sorts of the leaves         */
+    S512_PHASE_TRAIN = 9,       /* learned model training                   */
+    S512_NPHASE = 10
+};
+
+typedef struct s512_options {
+    uint32_t target_leaf;   /* mean leaf size the fan-out aims for (0: 2048 for 64-bit keys, 4096 for 32-bit) */
+    uint32_t oversample;    /* samples per splitter (0: 16)                  */
+    uint32_t max_fanout_log2; /* cap on log2 of the per-level fan-out, 1..9 (0: 9) */
+    uint32_t profile;       /* nonzero: time every kernel with HIP events    */
+    uint64_t seed;          /* sample positions (0: fixed default)           */
+} s512_options;
+
+typedef struct s512_stats {
+    int32_t levels;                     /* partition levels run              */
+    uint64_t tiles;                     /* leaf tiles sorted                 */
+    double kernel_ms[S512_NPHASE];      /* device time per phase (profile)   */
+    uint32_t launches[S512_NPHASE];     /* kernel launches per phase         */
+    uint64_t keys[S512_NPHASE];         /* keys processed per phase          */
+} s512_stats;
+
+/* Sort512::Sort with options and statistics (device pointers). */
+s512_status s512_sort_i32_dev_ex(int32_t* d_a, size_t n, const s512_options* opt, s512_stats* stats, void* stream);
+s512_status s512_sort_f64_dev_ex(double* d_a, size_t n, const s512_options* opt, s512_stats* stats, void* stream);
+
 /* -------------------------------------------------------------- key/value */
 /* Sort512kv::Sort<int,size_t>    sort512kv.hpp:6132-6135.  Keys ascending;
  * each value stays with its key; the order of values among equal keys is
  * unspecified (the reference is unstable too).                            */
 s512_status s512kv_sort_i32(int32_t* keys, int32_t* values, size_t n);
 s512_status s512kv_sort_i32_dev(int32_t* d_keys, int32_t* d_values, size_t n, void* stream);
+s512_status s512kv_sort_i32_dev_ex(int32_t* d_keys, int32_t* d_values, size_t n, const s512_options* opt,
+                                   s512_stats* stats, void* stream);
+/* Sort512kv::Partition512<size_t>   sort512kv.hpp:5958-6078                 */
+s512_status s512kv_partition_i32(int32_t* keys, int32_t* values, size_t left, size_t right, int32_t pivot,
+                                 size_t* out_bound);
+s512_status s512kv_partition_i32_dev(int32_t* d_keys, int32_t* d_values, size_t left, size_t right, int32_t pivot,
+                                     size_t* out_bound, void* stream);
+/* Sort512kv::SmallSort16V(int*, int*, len <= 256)   sort512kv.hpp:5235     */
+s512_status s512kv_smallsort16v_i32(int32_t* keys, int32_t* values, size_t len);
 
 #ifdef __cplusplus
 }

@@ -226,7 +226,6 @@ def test_kv_sort_closed_form(orc):
     orc.kv_sort(k, v)
     assert np.array_equal(k, np.sort(k0))
     assert np.array_equal(orc.pair_multiset(k, v), orc.pair_multiset(k0, v0))
-    assert np.array_equal(v, v0[np.argsort(k0, kind="stable")]) or True  # unstable: multiset is the contract
 
 
 def test_pair_multiset_independent(orc):
@@ -236,7 +235,8 @@ def test_pair_multiset_independent(orc):
     got = orc.pair_multiset(k, v)
     want = sorted(zip(k.tolist(), v.tolist()))
     dec = [(int(np.int32(np.uint32((x >> 32) ^ 0x80000000))), int(np.int32(np.uint32(x & 0xFFFFFFFF)))) for x in got]
-    assert dec == want
+    assert sorted(dec) == want
+    assert list(got) == sorted(got)
 
 
 # ---------------------------------------------------------------- merge
