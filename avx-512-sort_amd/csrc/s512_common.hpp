@@ -67,8 +67,12 @@ struct PartSeg {
     uint32_t nsample;    // samples drawn for the splitters
     uint64_t chunk;      // keys per chunk (last chunk may be short)
     uint64_t hist_base;  // offset of this segment's [nchunks][2F] histogram
-    uint64_t tree_base;  // offset of this segment's splitter tree / sorted splitters
+    uint64_t tree_base;  // offset of this segment's sorted splitters ([F] keys)
     uint64_t bkt_base;   // offset of this segment's [2F] bucket sizes / starts
+    uint64_t guide_base; // offset of this segment's guide table ([8F] cells)
+    uint64_t kmin;       // guide table origin (encoded key), set on the device
+    uint32_t shift;      // guide cell width 2^shift (encoded units), set on the device
+    uint32_t pad;
 };
 
 struct ChunkRef {

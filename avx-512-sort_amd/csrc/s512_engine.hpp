@@ -13,6 +13,7 @@ struct EngineParams {
     uint32_t max_chunk;      // keys per chunk, upper bound
     int max_levels;          // safety bound on partition levels
     int max_logf;            // cap on log2 fan-out per level
+    bool f64_minmax;         // fp64 leaves by v_min/v_max when the input allows
     uint64_t seed;           // sample positions
 };
 

@@ -74,22 +74,86 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wtot, 
 }
 
 // ---------------------------------------------------------------- classify
+// Splitters of a segment: srt[0 .. F-1) sorted and de-duplicated, padded with
+// KMAX, srt[F-1] = KMAX.  Bucket of key x: j = #{srt < x} ordinary bucket 2j,
+// or the equality bucket 2j+1 when x == srt[j].  j is found with a guide
+// table over the segment's encoded key range: cell c = (x - kmin) >> shift
+// (clamped) stores [j0, j1], the values j can take for keys in the cell, and a
+// binary search inside that range finishes (usually zero or one step).
+constexpr int GUIDE_PER_SPLIT = 8;
+constexpr int GMAX = GUIDE_PER_SPLIT * FMAX;
+
+struct Guide {
+    uint64_t kmin;
+    uint32_t shift;
+    uint32_t gmask;  // cells - 1
+};
+
 template <class K>
-__device__ __forceinline__ uint32_t classify(const K* tree, const K* srt, int logf, K x) {
-    uint32_t t = 1;
-    for (int l = 0; l < logf; ++l) t = 2 * t + (tree[t] < x ? 1u : 0u);
-    const uint32_t j = t - (1u << logf);     // number of splitters < x
-    return 2 * j + (srt[j] == x ? 1u : 0u);  // odd: equality bucket of splitter j
+__device__ __forceinline__ uint32_t guide_cell(const Guide& g, K x) {
+    if ((uint64_t)x < g.kmin) return 0;
+    const uint64_t c = ((uint64_t)x - g.kmin) >> g.shift;
+    return c > g.gmask ? g.gmask : (uint32_t)c;
+}
+
+template <class K>
+__device__ __forceinline__ uint32_t classify(const K* srt, const uint32_t* guide, const Guide& g, K x) {
+    const uint32_t e = guide[guide_cell(g, x)];
+    uint32_t lo = e & 0xFFFFu, hi = e >> 16;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (srt[mid] < x) lo = mid + 1;
+        else hi = mid;
+    }
+    return 2 * lo + (srt[lo] == x ? 1u : 0u);
+}
+
+template <class K>
+__device__ __forceinline__ uint32_t lower_bound_lds(const K* srt, uint32_t n, K x) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (srt[mid] < x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Load a segment's splitters and guide table into LDS.
+template <class K, int NT>
+__device__ __forceinline__ Guide load_classifier(const PartSeg& s, const K* trees, const uint32_t* guides, K* lsrt,
+                                                 uint32_t* lguide) {
+    const uint32_t F = 1u << s.logf, G = GUIDE_PER_SPLIT * F;
+    for (uint32_t j = threadIdx.x; j < F; j += NT) lsrt[j] = trees[s.tree_base + j];
+    for (uint32_t j = threadIdx.x; j < G; j += NT) lguide[j] = guides[s.guide_base + j];
+    return Guide{s.kmin, s.shift, G - 1};
 }
 
 // ---------------------------------------------------------------- tile kernels
+// How keys of type T enter and leave a network ordered by policy Pol.
+template <class T, class Pol>
+struct KeyIO {  // encoded unsigned keys
+    using K = typename Pol::key_t;
+    __device__ static __forceinline__ K load(T x) { return KeyTraits<T>::enc(x); }
+    __device__ static __forceinline__ T store(K k) { return KeyTraits<T>::dec(k); }
+    __device__ static __forceinline__ K pad() { return KeyTraits<T>::KMAX; }
+};
+template <>
+struct KeyIO<double, F64Order> {  // raw doubles (no NaN, no -0.0)
+    __device__ static __forceinline__ double load(double x) { return x; }
+    __device__ static __forceinline__ double store(double k) { return k; }
+    __device__ static __forceinline__ double pad() { return __builtin_huge_val(); }
+};
+template <class T>
+using DefaultOrder = UintOrder<typename KeyTraits<T>::K>;
+
 // Sort every listed segment (size <= P*E) in one workgroup; result goes to the
 // caller's array `a` at the same range.
-template <class T, int P, int E>
+template <class T, int P, int E, class Pol = DefaultOrder<T>>
 __global__ __launch_bounds__(P) void k_tile_sort(const TileSeg* segs, const uint32_t* count, T* a, const T* b) {
-    using Tr = KeyTraits<T>;
-    using K = typename Tr::K;
-    using BB = BlockBitonic<K, P, E>;
+    using IO = KeyIO<T, Pol>;
+    using K = typename Pol::key_t;
+    using BB = BlockBitonic<Pol, P, E>;
     __shared__ K lds[BB::N];
     const uint32_t cnt = *count;
     const uint32_t t = threadIdx.x;
@@ -100,7 +164,8 @@ __global__ __launch_bounds__(P) void k_tile_sort(const TileSeg* segs, const uint
 #pragma unroll
         for (int m = 0; m < E; ++m) {
             const uint32_t i = m * P + t;
-            lds[BB::phys(i)] = i < sg.size ? Tr::enc(src[i]) : Tr::KMAX;
+            const K x = IO::load(src[i < sg.size ? i : sg.size - 1]);
+            lds[BB::phys(i)] = i < sg.size ? x : IO::pad();
         }
         K v[E];
         BB::from_striped(v, lds, t);
@@ -109,17 +174,17 @@ __global__ __launch_bounds__(P) void k_tile_sort(const TileSeg* segs, const uint
 #pragma unroll
         for (int m = 0; m < E; ++m) {
             const uint32_t i = m * P + t;
-            if (i < sg.size) dst[i] = Tr::dec(lds[BB::phys(i)]);
+            if (i < sg.size) dst[i] = IO::store(lds[BB::phys(i)]);
         }
         __syncthreads();
     }
 }
 
 // One wave per segment (size <= 64*E), four waves per workgroup.
-template <class T, int E>
+template <class T, int E, class Pol = DefaultOrder<T>>
 __global__ __launch_bounds__(256) void k_wave_sort(const TileSeg* segs, const uint32_t* count, T* a, const T* b) {
-    using Tr = KeyTraits<T>;
-    using K = typename Tr::K;
+    using IO = KeyIO<T, Pol>;
+    using K = typename Pol::key_t;
     const uint32_t cnt = *count;
     const uint32_t lane = threadIdx.x & 63u;
     for (uint32_t s = blockIdx.x * 4 + (threadIdx.x >> 6); s < cnt; s += gridDim.x * 4) {
@@ -130,13 +195,14 @@ __global__ __launch_bounds__(256) void k_wave_sort(const TileSeg* segs, const ui
 #pragma unroll
         for (int r = 0; r < E; ++r) {
             const uint32_t i = lane * E + r;
-            v[r] = i < sg.size ? Tr::enc(src[i]) : Tr::KMAX;
+            const K x = IO::load(src[i < sg.size ? i : sg.size - 1]);
+            v[r] = i < sg.size ? x : IO::pad();
         }
-        WaveBitonic<K, E>::sort(v, lane);
+        WaveBitonic<Pol, E>::sort(v, lane);
 #pragma unroll
         for (int r = 0; r < E; ++r) {
             const uint32_t i = lane * E + r;
-            if (i < sg.size) dst[i] = Tr::dec(v[r]);
+            if (i < sg.size) dst[i] = IO::store(v[r]);
         }
     }
 }
@@ -151,7 +217,7 @@ __device__ __forceinline__ uint64_t kv_pack(int32_t k, uint32_t pos) {
 template <int P, int E>
 __global__ __launch_bounds__(P) void k_tile_sort_kv(const TileSeg* segs, const uint32_t* count, int32_t* ka,
                                                     int32_t* va, const int32_t* kb, const int32_t* vb) {
-    using BB = BlockBitonic<uint64_t, P, E>;
+    using BB = BlockBitonic<UintOrder<uint64_t>, P, E>;
     __shared__ uint64_t lds[BB::N];
     const uint32_t cnt = *count;
     const uint32_t t = threadIdx.x;
@@ -162,7 +228,8 @@ __global__ __launch_bounds__(P) void k_tile_sort_kv(const TileSeg* segs, const u
 #pragma unroll
         for (int m = 0; m < E; ++m) {
             const uint32_t i = m * P + t;
-            lds[BB::phys(i)] = i < sg.size ? kv_pack(sk[i], i) : ~0ull;
+            const uint64_t x = kv_pack(sk[i < sg.size ? i : sg.size - 1], i);
+            lds[BB::phys(i)] = i < sg.size ? x : ~0ull;
         }
         uint64_t v[E];
         BB::from_striped(v, lds, t);
@@ -204,9 +271,10 @@ __global__ __launch_bounds__(256) void k_wave_sort_kv(const TileSeg* segs, const
 #pragma unroll
         for (int r = 0; r < E; ++r) {
             const uint32_t i = lane * E + r;
-            v[r] = i < sg.size ? kv_pack(sk[i], i) : ~0ull;
+            const uint64_t x = kv_pack(sk[i < sg.size ? i : sg.size - 1], i);
+            v[r] = i < sg.size ? x : ~0ull;
         }
-        WaveBitonic<uint64_t, E>::sort(v, lane);
+        WaveBitonic<UintOrder<uint64_t>, E>::sort(v, lane);
         int32_t ov[E];
 #pragma unroll
         for (int r = 0; r < E; ++r) {
@@ -241,85 +309,105 @@ __global__ __launch_bounds__(256) void k_sample(const PartSeg* ps, const T* a, c
     }
 }
 
-// Splitters at evenly spaced ranks of the sorted sample; duplicates removed;
-// padded with KMAX to F-1 entries; laid out as an implicit (Eytzinger) tree.
-// trees[tree_base + 0 .. F)   : tree nodes 1..F-1 (slot 0 unused)
-// trees[tree_base + F .. 2F)  : sorted unique splitters, srt[F-1] = KMAX
+// Splitters at evenly spaced ranks of the sorted sample, duplicates removed,
+// padded with KMAX to F entries; then the guide table over [kmin, kmax] of
+// the sample.  One workgroup per segment.  Writes kmin/shift into the
+// segment descriptor for the count and scatter kernels.
 template <class K>
-__global__ __launch_bounds__(FMAX) void k_build_tree(const PartSeg* ps, const K* samples, uint32_t smax, K* trees) {
-    __shared__ K cand[FMAX];
+__global__ __launch_bounds__(FMAX) void k_build_tree(PartSeg* ps, const K* samples, uint32_t smax, K* trees,
+                                                     uint32_t* guides) {
+    __shared__ K srt[FMAX];
     __shared__ uint32_t wtot[FMAX / 64];
-    const PartSeg s = ps[blockIdx.x];
-    const uint32_t F = 1u << s.logf;
+    PartSeg& s = ps[blockIdx.x];
+    const uint32_t F = 1u << s.logf, G = GUIDE_PER_SPLIT * F;
+    const uint32_t ns = s.nsample;
     const K* smp = samples + (uint64_t)blockIdx.x * smax;
     const uint32_t t = threadIdx.x;
     K c = (K)~(K)0;
     bool keep = false;
     if (t + 1 < F) {
-        const uint32_t r = (uint32_t)(((uint64_t)(t + 1) * s.nsample) / F);
+        const uint32_t r = (uint32_t)(((uint64_t)(t + 1) * ns) / F);
         c = smp[r > 0 ? r - 1 : 0];
         keep = true;
         if (t > 0) {
-            const uint32_t rp = (uint32_t)(((uint64_t)t * s.nsample) / F);
+            const uint32_t rp = (uint32_t)(((uint64_t)t * ns) / F);
             keep = smp[rp > 0 ? rp - 1 : 0] != c;
         }
     }
     uint32_t nuniq;
     const uint32_t pos = block_excl_scan<FMAX>(keep ? 1u : 0u, wtot, &nuniq);
-    K* tree = trees + s.tree_base;
-    K* srt = tree + F;
     if (t < F) srt[t] = (K)~(K)0;
     __syncthreads();
     if (keep) srt[pos] = c;
     __syncthreads();
-    // copy to LDS, then fill the tree: node n at depth d, rank p in its level,
-    // holds the sorted element of in-order index ((2p+1) << (logf-1-d)) - 1.
-    if (t < F) cand[t] = srt[t];
-    __syncthreads();
-    if (t >= 1 && t < F) {
-        const int d = 31 - __clz(t);
-        const uint32_t p = t - (1u << d);
-        tree[t] = cand[((2 * p + 1) << (s.logf - 1 - d)) - 1];
+    K* out = trees + s.tree_base;
+    if (t < F) out[t] = srt[t];
+    // guide table over the sample's key range
+    const uint64_t kmin = (uint64_t)smp[0], kmax = (uint64_t)smp[ns - 1];
+    const uint64_t range = kmax - kmin;
+    uint32_t shift = 0;
+    while ((range >> shift) >= G) ++shift;
+    for (uint32_t cc = t; cc < G; cc += FMAX) {
+        const uint64_t off0 = (uint64_t)cc << shift, off1 = (uint64_t)(cc + 1) << shift;
+        const uint64_t kmaxk = (uint64_t)(K)~(K)0;
+        const K lo_key = (K)(off0 > kmaxk - kmin ? kmaxk : kmin + off0);
+        const K hi_key = (K)(off1 > kmaxk - kmin ? kmaxk : kmin + off1);
+        const uint32_t j0 = cc == 0 ? 0u : lower_bound_lds(srt, F - 1, lo_key);
+        const uint32_t j1 = cc == G - 1 ? F - 1 : lower_bound_lds(srt, F - 1, hi_key);
+        guides[s.guide_base + cc] = j0 | (j1 << 16);
     }
-    if (t == 0) tree[0] = (K)~(K)0;
+    if (t == 0) {
+        s.kmin = kmin;
+        s.shift = shift;
+    }
 }
 
 // ---------------------------------------------------------------- count
+// Encoded fp64 keys that are NaN or -0.0 (they rule out the min/max leaf sort).
+__device__ __forceinline__ bool nan_or_negzero(uint64_t k) {
+    return k == 0x7FFFFFFFFFFFFFFFull || k > 0xFFF0000000000000ull || k < 0x000FFFFFFFFFFFFFull;
+}
+__device__ __forceinline__ bool nan_or_negzero(uint32_t) { return false; }
+
 template <class T>
 __global__ __launch_bounds__(CNT_P) void k_count(const PartSeg* ps, const ChunkRef* cr, const T* a, const T* b,
-                                                 const typename KeyTraits<T>::K* trees, uint32_t* hist) {
+                                                 const typename KeyTraits<T>::K* trees, const uint32_t* guides,
+                                                 uint32_t* hist, uint32_t* special) {
     using Tr = KeyTraits<T>;
     using K = typename Tr::K;
-    __shared__ K ltree[FMAX];
     __shared__ K lsrt[FMAX];
+    __shared__ uint32_t lguide[GMAX];
     __shared__ uint32_t lh[BMAX];
     const ChunkRef c = cr[blockIdx.x];
     const PartSeg s = ps[c.seg];
     const uint32_t F = 1u << s.logf, B = 2 * F;
     const uint32_t t = threadIdx.x;
-    for (uint32_t j = t; j < F; j += CNT_P) {
-        ltree[j] = trees[s.tree_base + j];
-        lsrt[j] = trees[s.tree_base + F + j];
-    }
+    const Guide g = load_classifier<K, CNT_P>(s, trees, guides, lsrt, lguide);
     for (uint32_t j = t; j < B; j += CNT_P) lh[j] = 0;
     __syncthreads();
     const T* src = (s.buf ? b : a) + s.begin;
     const uint64_t lo = (uint64_t)c.idx * s.chunk;
     const uint64_t hi = lo + s.chunk < s.size ? lo + s.chunk : s.size;
-    const int logf = (int)s.logf;
+    bool odd = false;
     for (uint64_t base = lo; base < hi; base += (uint64_t)CNT_P * CNT_U) {
+        // clamped indices keep every load in bounds and unconditional, so all
+        // CNT_U loads of a thread are in flight together
         K x[CNT_U];
 #pragma unroll
         for (int u = 0; u < CNT_U; ++u) {
             const uint64_t i = base + (uint64_t)u * CNT_P + t;
-            x[u] = i < hi ? Tr::enc(src[i]) : (K)0;
+            x[u] = Tr::enc(src[i < hi ? i : hi - 1]);
         }
 #pragma unroll
         for (int u = 0; u < CNT_U; ++u) {
             const uint64_t i = base + (uint64_t)u * CNT_P + t;
-            if (i < hi) atomicAdd(&lh[classify(ltree, lsrt, logf, x[u])], 1u);
+            if (i < hi) {
+                odd |= nan_or_negzero(x[u]);
+                atomicAdd(&lh[classify(lsrt, lguide, g, x[u])], 1u);
+            }
         }
     }
+    if (__any(odd) && (t & 63u) == 0) atomicOr(special, 1u);
     __syncthreads();
     uint32_t* h = hist + s.hist_base + (uint64_t)c.idx * B;
     for (uint32_t j = t; j < B; j += CNT_P) h[j] = lh[j];
@@ -355,12 +443,12 @@ __global__ __launch_bounds__(CNT_P) void k_count(const PartSeg* ps, const ChunkR
 template <class T, bool KV>
 __global__ __launch_bounds__(SCAT_P) void k_scatter(const PartSeg* ps, const ChunkRef* cr, T* a, T* b,
                                                     int32_t* va, int32_t* vb,
-                                                    const typename KeyTraits<T>::K* trees, const uint32_t* hist,
-                                                    const uint64_t* bstart) {
+                                                    const typename KeyTraits<T>::K* trees, const uint32_t* guides,
+                                                    const uint32_t* hist, const uint64_t* bstart) {
     using Tr = KeyTraits<T>;
     using K = typename Tr::K;
-    __shared__ K ltree[FMAX];
     __shared__ K lsrt[FMAX];
+    __shared__ uint32_t lguide[GMAX];
     __shared__ uint32_t cursor[BMAX];
     __shared__ uint32_t rh[BMAX];
     __shared__ uint32_t rs[BMAX];
@@ -373,10 +461,7 @@ __global__ __launch_bounds__(SCAT_P) void k_scatter(const PartSeg* ps, const Chu
     const PartSeg s = ps[c.seg];
     const uint32_t F = 1u << s.logf, B = 2 * F;
     const uint32_t t = threadIdx.x;
-    for (uint32_t j = t; j < F; j += SCAT_P) {
-        ltree[j] = trees[s.tree_base + j];
-        lsrt[j] = trees[s.tree_base + F + j];
-    }
+    const Guide g = load_classifier<K, SCAT_P>(s, trees, guides, lsrt, lguide);
     const uint32_t* h = hist + s.hist_base + (uint64_t)c.idx * B;
     for (uint32_t j = t; j < B; j += SCAT_P) cursor[j] = (uint32_t)bstart[s.bkt_base + j] + h[j];
     const T* src = (s.buf ? b : a) + s.begin;
@@ -385,7 +470,6 @@ __global__ __launch_bounds__(SCAT_P) void k_scatter(const PartSeg* ps, const Chu
     int32_t* vdst = KV ? ((s.buf ? va : vb) + s.begin) : nullptr;
     const uint64_t lo = (uint64_t)c.idx * s.chunk;
     const uint64_t hi = lo + s.chunk < s.size ? lo + s.chunk : s.size;
-    const int logf = (int)s.logf;
 
     for (uint64_t base = lo; base < hi; base += SCAT_ROUND) {
         const uint32_t cnt = (uint32_t)((hi - base) < (uint64_t)SCAT_ROUND ? (hi - base) : (uint64_t)SCAT_ROUND);
@@ -397,16 +481,15 @@ __global__ __launch_bounds__(SCAT_P) void k_scatter(const PartSeg* ps, const Chu
 #pragma unroll
         for (int e = 0; e < SCAT_E; ++e) {
             const uint32_t i = e * SCAT_P + t;
-            if (i < cnt) {
-                x[e] = src[base + i];
-                if constexpr (KV) xv[e] = vsrc[base + i];
-            }
+            const uint64_t ic = base + (i < cnt ? i : cnt - 1);  // in bounds, unconditional
+            x[e] = src[ic];
+            if constexpr (KV) xv[e] = vsrc[ic];
         }
 #pragma unroll
         for (int e = 0; e < SCAT_E; ++e) {
             const uint32_t i = e * SCAT_P + t;
             if (i < cnt) {
-                bk[e] = classify(ltree, lsrt, logf, Tr::enc(x[e]));
+                bk[e] = classify(lsrt, lguide, g, Tr::enc(x[e]));
                 lr[e] = atomicAdd(&rh[bk[e]], 1u);
             }
         }
@@ -432,11 +515,18 @@ __global__ __launch_bounds__(SCAT_P) void k_scatter(const PartSeg* ps, const Chu
             }
         }
         __syncthreads();
-        for (uint32_t i = t; i < cnt; i += SCAT_P) {
-            const uint32_t bkt = stb[i];
+#pragma unroll
+        for (int e = 0; e < SCAT_E; ++e) {
+            const uint32_t i = e * SCAT_P + t;  // staged keys are grouped by bucket
+            const uint32_t bkt = stb[i] & (BMAX - 1u);  // slots >= cnt: stale, never stored
             const uint64_t d = (uint64_t)cursor[bkt] + (i - rs[bkt]);
-            dst[d] = stg[i];
-            if constexpr (KV) vdst[d] = stv[i];
+            const T y = stg[i];
+            int32_t yv = 0;
+            if constexpr (KV) yv = stv[i];
+            if (i < cnt) {
+                dst[d] = y;
+                if constexpr (KV) vdst[d] = yv;
+            }
         }
         __syncthreads();
         for (uint32_t j = t; j < B; j += SCAT_P) cursor[j] += rh[j];

@@ -36,7 +36,13 @@ __global__ __launch_bounds__(PART_P) void k_part_count(const T* a, uint64_t n, u
     const uint64_t lo = blockIdx.x * chunk;
     const uint64_t hi = lo + chunk < n ? lo + chunk : n;
     uint32_t c = 0;
-    for (uint64_t i = lo + threadIdx.x; i < hi; i += PART_P) c += le_pivot(a[i], pivot) ? 1u : 0u;
+    uint64_t i = lo + threadIdx.x;
+    for (; i + 3 * PART_P < hi; i += 4 * PART_P) {
+        const T x0 = a[i], x1 = a[i + PART_P], x2 = a[i + 2 * PART_P], x3 = a[i + 3 * PART_P];
+        c += (le_pivot(x0, pivot) ? 1u : 0u) + (le_pivot(x1, pivot) ? 1u : 0u) + (le_pivot(x2, pivot) ? 1u : 0u) +
+             (le_pivot(x3, pivot) ? 1u : 0u);
+    }
+    for (; i < hi; i += PART_P) c += le_pivot(a[i], pivot) ? 1u : 0u;
     for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d);
     if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = c;
     __syncthreads();
@@ -96,10 +102,9 @@ __global__ __launch_bounds__(PART_P) void k_part_scatter(const T* a, const int32
         for (int e = 0; e < PART_E; ++e) {
             const uint64_t i = base + (uint64_t)e * PART_P + t;
             const bool in = i < hi;
-            if (in) {
-                x[e] = a[i];
-                if constexpr (KV) xv[e] = va[i];
-            }
+            const uint64_t ic = in ? i : hi - 1;  // in bounds, unconditional load
+            x[e] = a[ic];
+            if constexpr (KV) xv[e] = va[ic];
             f[e] = in && le_pivot(x[e], pivot);
         }
 #pragma unroll

@@ -13,6 +13,7 @@
 #include "s512_kernels.hpp"
 #include "s512_runtime.hpp"
 #include "s512_engine.hpp"
+#include "s512_tile_f64.hpp"
 #include <algorithm>
 
 namespace s512 {
@@ -78,9 +79,12 @@ struct TileCfg {
 
 template <class T, bool KV>
 static void launch_tile(int cls, const TileSeg* segs, const uint32_t* dcount, uint32_t count, T* a, int32_t* va,
-                        const T* b, const int32_t* vb, hipStream_t st) {
+                        const T* b, const int32_t* vb, hipStream_t st, bool f64_fast = false) {
     if constexpr (KV) launch_tile_kv(cls, segs, dcount, count, a, va, b, vb, st);
-    else launch_tile_keys<T>(cls, segs, dcount, count, a, b, st);
+    else if constexpr (sizeof(T) == 8) {
+        if (f64_fast) launch_tile_f64_fast(cls, segs, dcount, count, a, b, st);
+        else launch_tile_keys<T>(cls, segs, dcount, count, a, b, st);
+    } else launch_tile_keys<T>(cls, segs, dcount, count, a, b, st);
 }
 
 static inline int ceil_log2(uint64_t x) {
@@ -153,6 +157,7 @@ s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const Engin
     int level = 0;
     uint64_t tiles = 0;
     uint64_t keys_ph[S512_NPHASE] = {0};
+    bool f64_fast = false;
     while (!segs.empty()) {
         if (++level > prm.max_levels) {
             set_error("partition did not converge within %d levels", prm.max_levels);
@@ -166,7 +171,7 @@ s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const Engin
         chunk = std::max<uint64_t>(SCAT_ROUND, std::min<uint64_t>(chunk, prm.max_chunk));
         ps.resize(m);
         crs.clear();
-        uint64_t histN = 0, sumB = 0, maxB = 0;
+        uint64_t histN = 0, sumB = 0, maxB = 0, sumF = 0, sumG = 0;
         for (uint32_t i = 0; i < m; ++i) {
             PartSeg& p = ps[i];
             p.begin = segs[i].begin;
@@ -180,8 +185,14 @@ s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const Engin
             p.chunk = chunk;
             p.nchunks = (uint32_t)((p.size + chunk - 1) / chunk);
             p.hist_base = histN;
-            p.tree_base = sumB;
+            p.tree_base = sumF;
             p.bkt_base = sumB;
+            p.guide_base = sumG;
+            p.kmin = 0;
+            p.shift = 0;
+            p.pad = 0;
+            sumF += F;
+            sumG += (uint64_t)GUIDE_PER_SPLIT * F;
             histN += (uint64_t)p.nchunks * B;
             sumB += B;
             maxB = std::max<uint64_t>(maxB, B);
@@ -198,7 +209,8 @@ s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const Engin
         sc.take<uint64_t>(16);
         const size_t up_bytes = sc.off;
         sc.take<K>((size_t)m * smax);
-        sc.take<K>(sumB);
+        sc.take<K>(sumF);
+        sc.take<uint32_t>(sumG);
         sc.take<uint32_t>(histN);
         sc.take<uint64_t>(sumB);
         sc.take<uint64_t>(sumB);
@@ -216,7 +228,8 @@ s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const Engin
         uint32_t* d_cnt = ar.take<uint32_t>(32);
         uint64_t* d_cnt64 = ar.take<uint64_t>(16);
         K* d_smp = ar.take<K>((size_t)m * smax);
-        K* d_tree = ar.take<K>(sumB);
+        K* d_tree = ar.take<K>(sumF);
+        uint32_t* d_guide = ar.take<uint32_t>(sumG);
         uint32_t* d_hist = ar.take<uint32_t>(histN);
         uint64_t* d_bstart = ar.take<uint64_t>(sumB);
         uint64_t* d_bsize = ar.take<uint64_t>(sumB);
@@ -257,11 +270,11 @@ s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const Engin
         S512_CHECK_LAUNCH();
         pf.end();
         pf.begin(S512_PHASE_TREE);
-        k_build_tree<K><<<m, FMAX, 0, st>>>(d_ps, d_smp, smax, d_tree);
+        k_build_tree<K><<<m, FMAX, 0, st>>>(d_ps, d_smp, smax, d_tree, d_guide);
         S512_CHECK_LAUNCH();
         pf.end();
         pf.begin(S512_PHASE_COUNT);
-        k_count<T><<<tc, CNT_P, 0, st>>>(d_ps, d_cr, a, b, d_tree, d_hist);
+        k_count<T><<<tc, CNT_P, 0, st>>>(d_ps, d_cr, a, b, d_tree, d_guide, d_hist, d_cnt + 11);
         S512_CHECK_LAUNCH();
         pf.end();
         pf.begin(S512_PHASE_SCAN);
@@ -269,7 +282,7 @@ s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const Engin
         S512_CHECK_LAUNCH();
         pf.end();
         pf.begin(S512_PHASE_SCATTER);
-        k_scatter<T, KV><<<tc, SCAT_P, 0, st>>>(d_ps, d_cr, a, b, va, vb, d_tree, d_hist, d_bstart);
+        k_scatter<T, KV><<<tc, SCAT_P, 0, st>>>(d_ps, d_cr, a, b, va, vb, d_tree, d_guide, d_hist, d_bstart);
         S512_CHECK_LAUNCH();
         pf.end();
         pf.begin(S512_PHASE_PLAN);
@@ -285,6 +298,8 @@ s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const Engin
         uint64_t counts64[16];
         memcpy(counts, h_out, sizeof(counts));
         memcpy(counts64, (char*)h_out + ((char*)d_cnt64 - (char*)d_cnt), sizeof(counts64));
+        // level 1 saw every key: fp64 leaves may use min/max when no key is NaN or -0.0
+        if (level == 1) f64_fast = !KV && sizeof(T) == 8 && prm.f64_minmax && counts[11] == 0;
         if (counts[8]) {
             pf.begin(S512_PHASE_COPY);
             k_copy_back<T, KV><<<std::min<uint32_t>(counts[8], 16384), 256, 0, st>>>(pl.copy, pl.copy_count, a, b, va,
@@ -296,7 +311,7 @@ s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const Engin
         for (int c = 0; c < ncls; ++c) {
             if (!counts[c]) continue;
             pf.begin(S512_PHASE_TILE);
-            launch_tile<T, KV>(c, pl.tile[c], d_cnt + c, counts[c], a, va, b, vb, st);
+            launch_tile<T, KV>(c, pl.tile[c], d_cnt + c, counts[c], a, va, b, vb, st, f64_fast);
             S512_CHECK_LAUNCH();
             pf.end();
             tiles += counts[c];
@@ -338,6 +353,7 @@ EngineParams default_params(bool wide) {
     p.max_levels = 48;
     p.max_logf = FMAX_LOG;
     p.seed = 0x5EEDull;
+    p.f64_minmax = true;
     return p;
 }
 

@@ -40,6 +40,36 @@ __device__ __forceinline__ void cex_up(K& a, K& b) {
     b = y;
 }
 
+// ---------------------------------------------------------------- ordering policies
+// A policy supplies the compare-exchange (ascending) and the order-reversing
+// "flip" used for descending blocks: every merge phase flips the keys of the
+// blocks that the bitonic network sorts descending, runs ascending
+// compare-exchanges only, and unflips at the end of the phase.
+template <class K>
+struct UintOrder {  // encoded keys: bitwise NOT reverses the unsigned order
+    using key_t = K;
+    __device__ static __forceinline__ void ce(K& a, K& b) { cex_up(a, b); }
+    __device__ static __forceinline__ K flip(K x, bool f) { return f ? (K)~x : x; }
+    __device__ static __forceinline__ bool less(K a, K b) { return a < b; }
+};
+// Raw doubles that contain no NaN and no -0.0 (the caller checks): IEEE
+// min/max give the exact ascending pair, negation reverses the order.  Only
+// compiled into translation units built with -fno-honor-nans, where min/max
+// lower to single v_min_f64 / v_max_f64 instructions.
+struct F64Order {
+    using key_t = double;
+    __device__ static __forceinline__ void ce(double& a, double& b) {
+        const double lo = __builtin_fmin(a, b), hi = __builtin_fmax(a, b);
+        a = lo;
+        b = hi;
+    }
+    __device__ static __forceinline__ double flip(double x, bool f) {
+        const uint64_t u = __builtin_bit_cast(uint64_t, x) ^ (f ? 0x8000000000000000ull : 0ull);
+        return __builtin_bit_cast(double, u);
+    }
+    __device__ static __forceinline__ bool less(double a, double b) { return a < b; }
+};
+
 // ---------------------------------------------------------------- lane exchange
 template <int M>
 __device__ __forceinline__ uint32_t xor_lane32(uint32_t v) {
@@ -61,68 +91,64 @@ __device__ __forceinline__ uint64_t xor_lane(uint64_t v) {
     const uint32_t hi = xor_lane32<M>((uint32_t)(v >> 32));
     return ((uint64_t)hi << 32) | lo;
 }
+template <int M>
+__device__ __forceinline__ double xor_lane(double v) {
+    return __builtin_bit_cast(double, xor_lane<M>(__builtin_bit_cast(uint64_t, v)));
+}
 
 // ---------------------------------------------------------------- in-register stages
-// Stage along register bit PB of all E registers, direction per register given
-// by bit q of the element index (i0 | r << w).
-template <class K, int E, int PB>
-__device__ __forceinline__ void reg_stage(K (&v)[E], uint32_t i0, int w, int q, int logn) {
+// Flip every register of this thread (all of them belong to blocks of the same
+// direction in the layouts where this is called).
+template <class Pol, int E>
+__device__ __forceinline__ void flip_all(typename Pol::key_t (&v)[E], bool f) {
+#pragma unroll
+    for (int r = 0; r < E; ++r) v[r] = Pol::flip(v[r], f);
+}
+// Ascending compare-exchange along register bit PB.
+template <class Pol, int E, int PB>
+__device__ __forceinline__ void reg_stage_up(typename Pol::key_t (&v)[E]) {
 #pragma unroll
     for (int r = 0; r < E; ++r) {
         if (r & (1 << PB)) continue;
-        const uint32_t ir = i0 | ((uint32_t)r << w);
-        const bool desc = (q < logn) && ((ir >> q) & 1u);
-        cex_dir(v[r], v[r | (1 << PB)], desc);
+        Pol::ce(v[r], v[r | (1 << PB)]);
     }
 }
-template <class K, int E, int PB>
-__device__ __forceinline__ void reg_stage_u(K (&v)[E], bool desc) {
-#pragma unroll
-    for (int r = 0; r < E; ++r) {
-        if (r & (1 << PB)) continue;
-        cex_dir(v[r], v[r | (1 << PB)], desc);
+// Stages PB, PB-1, .., 0 ascending.
+template <class Pol, int E, int PB>
+__device__ __forceinline__ void reg_low_up(typename Pol::key_t (&v)[E]) {
+    if constexpr (PB >= 0) {
+        reg_stage_up<Pol, E, PB>(v);
+        reg_low_up<Pol, E, PB - 1>(v);
     }
 }
-template <class K, int E>
-__device__ __forceinline__ void reg_stage_any(K (&v)[E], int pb, uint32_t i0, int w, int q, int logn) {
-    if constexpr (E >= 2) { if (pb == 0) { reg_stage<K, E, 0>(v, i0, w, q, logn); return; } }
-    if constexpr (E >= 4) { if (pb == 1) { reg_stage<K, E, 1>(v, i0, w, q, logn); return; } }
-    if constexpr (E >= 8) { if (pb == 2) { reg_stage<K, E, 2>(v, i0, w, q, logn); return; } }
-    if constexpr (E >= 16) { if (pb == 3) { reg_stage<K, E, 3>(v, i0, w, q, logn); return; } }
-    if constexpr (E >= 32) { if (pb == 4) { reg_stage<K, E, 4>(v, i0, w, q, logn); return; } }
-}
-// Low bits of the blocked layout with a direction uniform over the registers.
-template <class K, int E>
-__device__ __forceinline__ void reg_low_stages(K (&v)[E], bool desc) {
-    if constexpr (E >= 32) reg_stage_u<K, E, 4>(v, desc);
-    if constexpr (E >= 16) reg_stage_u<K, E, 3>(v, desc);
-    if constexpr (E >= 8) reg_stage_u<K, E, 2>(v, desc);
-    if constexpr (E >= 4) reg_stage_u<K, E, 1>(v, desc);
-    if constexpr (E >= 2) reg_stage_u<K, E, 0>(v, desc);
-}
-// Bitonic sort of the E registers of one thread: merge sizes 2..E; the
-// direction of merge size 2^q is bit q of the global index t*E + r, i.e. a
-// register bit for q < log2 E and bit 0 of `tbit` for q == log2 E.
-template <class K, int E>
-__device__ __forceinline__ void reg_sort(K (&v)[E], uint32_t tbit) {
+// Bitonic sort of the E registers of one thread, blocked layout (element
+// t*E + r).  Merge sizes 2^q < E have register-determined directions (the
+// operands are swapped at compile time); merge size E has direction bit 0 of
+// `tbit`, handled by a flip.  On exit the registers are flipped iff
+// `flip_last` (the caller's next phase unflips them).
+template <class Pol, int E>
+__device__ __forceinline__ void reg_sort(typename Pol::key_t (&v)[E], bool flip_last) {
     constexpr int LOGE = ilog2(E);
 #pragma unroll
-    for (int q = 1; q <= LOGE; ++q) {
+    for (int q = 1; q < LOGE; ++q) {
 #pragma unroll
         for (int b = q - 1; b >= 0; --b) {
 #pragma unroll
             for (int r = 0; r < E; ++r) {
                 if (r & (1 << b)) continue;
-                const bool desc = (q < LOGE) ? (((r >> q) & 1) != 0) : ((tbit & 1u) != 0);
-                cex_dir(v[r], v[r | (1 << b)], desc);
+                if ((r >> q) & 1) Pol::ce(v[r | (1 << b)], v[r]);
+                else Pol::ce(v[r], v[r | (1 << b)]);
             }
         }
     }
+    flip_all<Pol, E>(v, flip_last);
+    reg_low_up<Pol, E, LOGE - 1>(v);
 }
 
 // ---------------------------------------------------------------- workgroup tile
-template <class K, int P, int E>
+template <class Pol, int P, int E>
 struct BlockBitonic {
+    using K = typename Pol::key_t;
     static constexpr int LOGE = ilog2(E);
     static constexpr int LOGP = ilog2(P);
     static constexpr int N = P * E;
@@ -135,8 +161,28 @@ struct BlockBitonic {
         const uint32_t lm = (1u << w) - 1u;
         return ((t & ~lm) << LOGE) | (r << w) | (t & lm);
     }
-    // XOR swizzle so that the blocked layout does not put a wave on one bank.
-    __device__ static __forceinline__ uint32_t phys(uint32_t i) { return i ^ ((i >> LOGE) & (E - 1)); }
+    // LDS position of element i: a GF(2)-linear bijection (each output bit is
+    // XORed only with higher input bits) chosen so that, in every layout the
+    // network uses (striped, blocked and every window) and for every register,
+    // the lanes of a wave touch distinct banks -- both when 32 consecutive
+    // lanes of 8-byte keys (64 lanes of 4-byte keys) are served per cycle and
+    // when 16 (32) are.  Found by exhaustive search over XOR taps; for the
+    // configurations below every access is conflict-free in both models.
+    __device__ static __forceinline__ uint32_t phys(uint32_t i) {
+        if constexpr (sizeof(K) == 8 && E == 16) {
+            return i ^ ((i >> 4) & 31u);
+        } else if constexpr (sizeof(K) == 8 && E == 8) {
+            return i ^ ((i >> 4) & 3u) ^ (((i >> 6) & 1u) * 12u) ^ (((i >> 7) & 1u) << 4);
+        } else if constexpr (sizeof(K) == 8 && E == 4) {
+            return i ^ (((i >> 4) & 1u) * 5u) ^ (((i >> 5) & 1u) * 10u) ^ (((i >> 6) & 1u) * 16u);
+        } else if constexpr (sizeof(K) == 4 && E == 8) {
+            return i ^ ((i >> 5) & 1u) ^ (((i >> 6) & 1u) * 10u) ^ (((i >> 7) & 1u) * 20u) ^ (((i >> 8) & 1u) * 32u);
+        } else if constexpr (sizeof(K) == 4 && E == 16) {
+            return i ^ ((i >> 5) & 7u) ^ (((i >> 8) & 1u) * 24u) ^ (((i >> 9) & 1u) * 32u);
+        } else {
+            return i ^ ((i >> LOGE) & (E - 1));
+        }
+    }
 
     template <int WF, int WT>
     __device__ static __forceinline__ void relayout(K (&v)[E], K* lds, uint32_t t) {
@@ -148,66 +194,50 @@ struct BlockBitonic {
         __syncthreads();
     }
 
-    // Stages on index bits B, B-1, .., W of merge size 2^Q while the window
-    // [W, W+LOGE) is in the registers.  The direction of a pair is bit Q of
-    // its index: a register bit when Q lies inside the window, otherwise bit
-    // Q-LOGE of the thread id (or ascending for the final merge Q == LOGN).
-    template <int Q, int W, int B>
-    __device__ static __forceinline__ void win_stages(K (&v)[E], uint32_t t) {
+    // Stages on index bits B, B-1, .., W while the window [W, W+LOGE) is in
+    // the registers; all ascending (descending blocks are flipped).
+    template <int W, int B>
+    __device__ static __forceinline__ void win_stages(K (&v)[E]) {
         if constexpr (B >= W) {
-            constexpr int PB = B - W;
-            if constexpr (Q < W + LOGE) {
-#pragma unroll
-                for (int r = 0; r < E; ++r) {
-                    if (r & (1 << PB)) continue;
-                    if ((r >> (Q - W)) & 1) cex_up(v[r | (1 << PB)], v[r]);
-                    else cex_up(v[r], v[r | (1 << PB)]);
-                }
-            } else if constexpr (Q == LOGN) {
-#pragma unroll
-                for (int r = 0; r < E; ++r) {
-                    if (r & (1 << PB)) continue;
-                    cex_up(v[r], v[r | (1 << PB)]);
-                }
-            } else {
-                const bool desc = ((t >> (Q - LOGE)) & 1u) != 0;
-#pragma unroll
-                for (int r = 0; r < E; ++r) {
-                    if (r & (1 << PB)) continue;
-                    cex_dir(v[r], v[r | (1 << PB)], desc);
-                }
-            }
-            win_stages<Q, W, B - 1>(v, t);
+            reg_stage_up<Pol, E, B - W>(v);
+            win_stages<W, B - 1>(v);
         }
     }
 
-    // All stages of merge size 2^Q above the register bits, window by window
+    // All stages of one merge phase above the register bits, window by window
     // from the top bit down, then back to the blocked layout for the rest.
-    template <int Q, int HI, int WPREV>
+    template <int HI, int WPREV>
     __device__ static __forceinline__ void windows(K (&v)[E], K* lds, uint32_t t) {
         if constexpr (HI >= LOGE) {
             constexpr int NW = (HI - LOGE + 1) > LOGE ? (HI - LOGE + 1) : LOGE;
             relayout<WPREV, NW>(v, lds, t);
-            win_stages<Q, NW, HI>(v, t);
-            windows<Q, NW - 1, NW>(v, lds, t);
+            win_stages<NW, HI>(v);
+            windows<NW - 1, NW>(v, lds, t);
         } else {
             relayout<WPREV, 0>(v, lds, t);
-            win_stages<Q, 0, LOGE - 1>(v, t);
+            win_stages<0, LOGE - 1>(v);
         }
     }
 
+    // Merge phase 2^Q: in the blocked layout the direction of element
+    // t*E + r is bit Q of its index, i.e. bit Q-LOGE of t (ascending for the
+    // final phase).  `prev` says whether the registers are flipped for the
+    // previous phase.
     template <int Q>
-    __device__ static __forceinline__ void merges(K (&v)[E], K* lds, uint32_t t) {
+    __device__ static __forceinline__ void merges(K (&v)[E], K* lds, uint32_t t, bool prev) {
         if constexpr (Q <= LOGN) {
-            windows<Q, Q - 1, 0>(v, lds, t);
-            merges<Q + 1>(v, lds, t);
+            const bool cur = (Q < LOGN) && (((t >> (Q - LOGE)) & 1u) != 0);
+            flip_all<Pol, E>(v, prev != cur);
+            windows<Q - 1, 0>(v, lds, t);
+            merges<Q + 1>(v, lds, t, cur);
         }
     }
 
     // v holds the blocked layout on entry and on exit; sorted ascending.
     __device__ static void sort(K (&v)[E], K* lds, uint32_t t) {
-        reg_sort<K, E>(v, t);
-        merges<LOGE + 1>(v, lds, t);
+        const bool f = (t & 1u) != 0;  // direction of merge size E
+        reg_sort<Pol, E>(v, f);
+        merges<LOGE + 1>(v, lds, t, f);
     }
 
     // striped global -> blocked registers, through LDS
@@ -225,43 +255,47 @@ struct BlockBitonic {
 };
 
 // ---------------------------------------------------------------- wave tile
-template <class K, int E>
+template <class Pol, int E>
 struct WaveBitonic {
+    using K = typename Pol::key_t;
     static constexpr int LOGE = ilog2(E);
     static constexpr int N = 64 * E;
     static constexpr int LOGN = LOGE + 6;
 
+    // ascending stage between lanes l and l^M: the lower lane keeps the min
     template <int M>
-    __device__ static __forceinline__ void lane_stage(K (&v)[E], uint32_t lane, bool desc) {
+    __device__ static __forceinline__ void lane_stage(K (&v)[E], uint32_t lane) {
         const bool upper = (lane & M) != 0;
-        const bool keep_min = (upper == desc);
 #pragma unroll
         for (int r = 0; r < E; ++r) {
             const K p = xor_lane<M>(v[r]);
-            const bool take = keep_min ? (p < v[r]) : (v[r] < p);
-            v[r] = take ? p : v[r];
+            K lo = v[r], hi = p;
+            Pol::ce(lo, hi);
+            v[r] = upper ? hi : lo;
         }
     }
-    __device__ static __forceinline__ void lane_stage_any(K (&v)[E], int lb, uint32_t lane, bool desc) {
-        switch (lb) {
-            case 0: lane_stage<1>(v, lane, desc); break;
-            case 1: lane_stage<2>(v, lane, desc); break;
-            case 2: lane_stage<4>(v, lane, desc); break;
-            case 3: lane_stage<8>(v, lane, desc); break;
-            case 4: lane_stage<16>(v, lane, desc); break;
-            default: lane_stage<32>(v, lane, desc); break;
+    template <int LB>
+    __device__ static __forceinline__ void lane_stages_down(K (&v)[E], uint32_t lane) {
+        if constexpr (LB >= 0) {
+            lane_stage<(1 << LB)>(v, lane);
+            lane_stages_down<LB - 1>(v, lane);
+        }
+    }
+    template <int Q>
+    __device__ static __forceinline__ void merges(K (&v)[E], uint32_t lane, bool prev) {
+        if constexpr (Q <= LOGN) {
+            const bool cur = (Q < LOGN) && (((lane >> (Q - LOGE)) & 1u) != 0);
+            flip_all<Pol, E>(v, prev != cur);
+            lane_stages_down<Q - 1 - LOGE>(v, lane);
+            reg_low_up<Pol, E, LOGE - 1>(v);
+            merges<Q + 1>(v, lane, cur);
         }
     }
     // blocked layout: lane l holds keys l*E .. l*E+E-1; sorted ascending.
     __device__ static void sort(K (&v)[E], uint32_t lane) {
-        reg_sort<K, E>(v, lane);
-#pragma unroll
-        for (int q = LOGE + 1; q <= LOGN; ++q) {
-            const bool desc = (q < LOGN) && (((lane >> (q - LOGE)) & 1u) != 0);
-#pragma unroll
-            for (int b = q - 1; b >= LOGE; --b) lane_stage_any(v, b - LOGE, lane, desc);
-            reg_low_stages<K, E>(v, desc);
-        }
+        const bool f = (lane & 1u) != 0;
+        reg_sort<Pol, E>(v, f);
+        merges<LOGE + 1>(v, lane, f);
     }
 };
 

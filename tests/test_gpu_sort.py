@@ -143,3 +143,42 @@ def test_kv_intmax_keys_keep_their_values(s512, orc, dev):
         tk, tv = to_dev(k0, dev), to_dev(v0, dev)
         s512.kv_sort(tk, tv)
         _kv_check(orc, k0, v0, to_host(tk), to_host(tv), f"INT_MAX n={n}")
+
+
+def _enc_sorted_f64(a):
+    """Ascending by the library's total order on doubles: IEEE order, -0.0
+    before +0.0, negative NaNs first and positive NaNs last (by bit pattern)."""
+    u = a.view(np.uint64)
+    key = np.where((u >> np.uint64(63)) == 1, ~u, u | np.uint64(1 << 63))
+    return a[np.argsort(key, kind="stable")]
+
+
+@pytest.mark.parametrize("n", [100, 9000, 300_000, 2_000_000])
+def test_sort_f64_nan_negzero_fallback(s512, orc, dev, n):
+    """NaN and -0.0 keys: the min/max leaf kernels must not run; the encoded
+    integer path gives the documented total order bit for bit."""
+    a = orc.gen_f64("normal", n, 21)
+    a[::97] = np.nan
+    a[5::101] = -np.nan
+    a[7::89] = -0.0
+    a[9::83] = 0.0
+    want = _enc_sorted_f64(a.copy())
+    t = to_dev(a, dev)
+    s512.sort(t)
+    got = to_host(t)
+    assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
+
+
+@pytest.mark.parametrize("n", [50_000, 5_000_000])
+def test_sort_f64_subnormal_inf_fast_path(s512, orc, dev, n):
+    """No NaN / -0.0, so the v_min_f64/v_max_f64 leaves run: subnormals,
+    +-inf and +0.0 must come back bit for bit."""
+    a = orc.gen_f64("normal", n, 22) * 1e-310
+    a[::13] = np.inf
+    a[1::17] = -np.inf
+    a[2::19] = 0.0
+    a[3::23] = np.finfo(np.float64).max
+    want = np.sort(a)
+    t = to_dev(a, dev)
+    s512.sort(t)
+    assert np.array_equal(to_host(t).view(np.uint64), want.view(np.uint64))
