@@ -72,7 +72,9 @@ struct PartSeg {
     uint64_t guide_base; // offset of this segment's guide table ([8F] cells)
     uint64_t kmin;       // guide table origin (encoded key), set on the device
     uint32_t shift;      // guide cell width 2^shift (encoded units), set on the device
-    uint32_t pad;
+    uint32_t mode;       // 0: sampled splitters, 1: learned model
+    uint32_t lshift;     // learned: bucket = (fine bucket >> lshift) - lbase
+    uint32_t lbase;
 };
 
 struct ChunkRef {
@@ -80,13 +82,6 @@ struct ChunkRef {
     uint32_t idx;
 };
 
-// Learned-model descriptor (monotone CDF tabulated on a grid): see learned.hip.
-struct CdfTable {
-    double lo;        // key value mapped to grid coordinate 0
-    double scale;     // grid cells per unit key
-    uint32_t ncell;   // grid has ncell+1 knots
-    uint32_t pad;
-};
 
 __host__ __device__ constexpr int ilog2(unsigned v) { return v <= 1 ? 0 : 1 + ilog2(v >> 1); }
 

@@ -14,6 +14,11 @@ struct EngineParams {
     int max_levels;          // safety bound on partition levels
     int max_logf;            // cap on log2 fan-out per level
     bool f64_minmax;         // fp64 leaves by v_min/v_max when the input allows
+    bool learned;            // learned CDF partitioning for the first two levels
+    uint32_t sample_permille;// learned: sample size (per mille of n)
+    uint32_t train_steps;    // learned: Adam steps
+    float learn_rate;        // learned: Adam step size
+    float imbalance;         // learned: fall back when a level-1 bucket exceeds this x mean
     uint64_t seed;           // sample positions
 };
 
@@ -24,5 +29,11 @@ EngineParams params_from(const s512_options* opt, bool wide_keys);
 template <class T, bool KV>
 s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const EngineParams& prm, s512_stats* stats,
                         bool profile);
+
+struct DeviceState;
+// Same, with the device workspace lock already held by the caller.
+template <class T, bool KV>
+s512_status sort_engine_locked(T* a, int32_t* va, size_t n, hipStream_t st, const EngineParams& prm,
+                               s512_stats* stats, bool profile, DeviceState& ds);
 
 }  // namespace s512

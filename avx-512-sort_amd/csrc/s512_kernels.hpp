@@ -21,6 +21,7 @@
 #pragma once
 #include "s512_common.hpp"
 #include "s512_tile.hpp"
+#include "s512_learned.hpp"
 
 namespace s512 {
 
@@ -119,14 +120,34 @@ __device__ __forceinline__ uint32_t lower_bound_lds(const K* srt, uint32_t n, K 
     return lo;
 }
 
-// Load a segment's splitters and guide table into LDS.
+// Load a segment's classifier into LDS: splitters + guide table, or (learned
+// segments) the tabulated model in the guide table's place.
+constexpr int LDS_TABLE_WORDS = GMAX > LEARN_G + 1 ? GMAX : LEARN_G + 1;
 template <class K, int NT>
 __device__ __forceinline__ Guide load_classifier(const PartSeg& s, const K* trees, const uint32_t* guides, K* lsrt,
-                                                 uint32_t* lguide) {
+                                                 uint32_t* lguide, const LearnedTable& lt) {
+    if (s.mode) {
+        for (uint32_t j = threadIdx.x; j <= LEARN_G; j += NT) lguide[j] = __float_as_uint(lt.tab[j]);
+        return Guide{0, 0, 0};
+    }
     const uint32_t F = 1u << s.logf, G = GUIDE_PER_SPLIT * F;
     for (uint32_t j = threadIdx.x; j < F; j += NT) lsrt[j] = trees[s.tree_base + j];
     for (uint32_t j = threadIdx.x; j < G; j += NT) lguide[j] = guides[s.guide_base + j];
     return Guide{s.kmin, s.shift, G - 1};
+}
+
+// Bucket of key x in segment s (learned segments use only even buckets).
+template <class K>
+__device__ __forceinline__ uint32_t classify_seg(const PartSeg& s, const K* lsrt, const uint32_t* lguide,
+                                                 const Guide& g, const LearnedTable& lt, K x) {
+    if (s.mode) {
+        const uint32_t fb = learned_fine_bucket((const float*)lguide, lt.kmin, lt.inv_range, lt.nb_log, x);
+        int b = (int)(fb >> s.lshift) - (int)s.lbase;
+        const int fmax = (1 << s.logf) - 1;
+        b = b < 0 ? 0 : (b > fmax ? fmax : b);
+        return 2u * (uint32_t)b;
+    }
+    return classify(lsrt, lguide, g, x);
 }
 
 // ---------------------------------------------------------------- tile kernels
@@ -319,6 +340,7 @@ __global__ __launch_bounds__(FMAX) void k_build_tree(PartSeg* ps, const K* sampl
     __shared__ K srt[FMAX];
     __shared__ uint32_t wtot[FMAX / 64];
     PartSeg& s = ps[blockIdx.x];
+    if (s.mode || s.nsample == 0) return;  // learned segment: no splitters
     const uint32_t F = 1u << s.logf, G = GUIDE_PER_SPLIT * F;
     const uint32_t ns = s.nsample;
     const K* smp = samples + (uint64_t)blockIdx.x * smax;
@@ -372,17 +394,17 @@ __device__ __forceinline__ bool nan_or_negzero(uint32_t) { return false; }
 template <class T>
 __global__ __launch_bounds__(CNT_P) void k_count(const PartSeg* ps, const ChunkRef* cr, const T* a, const T* b,
                                                  const typename KeyTraits<T>::K* trees, const uint32_t* guides,
-                                                 uint32_t* hist, uint32_t* special) {
+                                                 uint32_t* hist, uint32_t* special, LearnedTable lt) {
     using Tr = KeyTraits<T>;
     using K = typename Tr::K;
     __shared__ K lsrt[FMAX];
-    __shared__ uint32_t lguide[GMAX];
+    __shared__ uint32_t lguide[LDS_TABLE_WORDS];
     __shared__ uint32_t lh[BMAX];
     const ChunkRef c = cr[blockIdx.x];
     const PartSeg s = ps[c.seg];
     const uint32_t F = 1u << s.logf, B = 2 * F;
     const uint32_t t = threadIdx.x;
-    const Guide g = load_classifier<K, CNT_P>(s, trees, guides, lsrt, lguide);
+    const Guide g = load_classifier<K, CNT_P>(s, trees, guides, lsrt, lguide, lt);
     for (uint32_t j = t; j < B; j += CNT_P) lh[j] = 0;
     __syncthreads();
     const T* src = (s.buf ? b : a) + s.begin;
@@ -403,7 +425,7 @@ __global__ __launch_bounds__(CNT_P) void k_count(const PartSeg* ps, const ChunkR
             const uint64_t i = base + (uint64_t)u * CNT_P + t;
             if (i < hi) {
                 odd |= nan_or_negzero(x[u]);
-                atomicAdd(&lh[classify(lsrt, lguide, g, x[u])], 1u);
+                atomicAdd(&lh[classify_seg(s, lsrt, lguide, g, lt, x[u])], 1u);
             }
         }
     }
@@ -439,20 +461,31 @@ __global__ __launch_bounds__(CNT_P) void k_count(const PartSeg* ps, const ChunkR
 }
 
 // ---------------------------------------------------------------- scatter
-// Keys (and values when V != void-like) move from array buf to array 1-buf.
+// Keys (and values for key/value sorts) move from array buf to array 1-buf.
+// Per round of SCAT_ROUND keys:
+//   classify, rank inside the round with LDS atomics           | barrier 1
+//   wave-level scan of the round histogram (2 buckets/thread)  | barrier 2
+//   bucket owners: round start rs, destination base, advance the
+//   chunk's cursor, clear the counter for the next round       | barrier 3
+//   stage keys grouped by bucket in LDS                        | barrier 4
+//   write the staged keys: consecutive threads, consecutive addresses
+// The next round's keys are loaded into registers while this round runs.
 template <class T, bool KV>
 __global__ __launch_bounds__(SCAT_P) void k_scatter(const PartSeg* ps, const ChunkRef* cr, T* a, T* b,
                                                     int32_t* va, int32_t* vb,
                                                     const typename KeyTraits<T>::K* trees, const uint32_t* guides,
-                                                    const uint32_t* hist, const uint64_t* bstart) {
+                                                    const uint32_t* hist, const uint64_t* bstart, LearnedTable lt) {
     using Tr = KeyTraits<T>;
     using K = typename Tr::K;
+    constexpr int NW = SCAT_P / 64;
+    static_assert(2 * SCAT_P == BMAX, "scan: two buckets per thread");
     __shared__ K lsrt[FMAX];
-    __shared__ uint32_t lguide[GMAX];
-    __shared__ uint32_t cursor[BMAX];
-    __shared__ uint32_t rh[BMAX];
-    __shared__ uint32_t rs[BMAX];
-    __shared__ uint32_t wtot[SCAT_P / 64];
+    __shared__ uint32_t lguide[LDS_TABLE_WORDS];
+    __shared__ uint32_t cursor[BMAX];  // next destination of each bucket (segment-relative)
+    __shared__ uint32_t rh[BMAX];      // round histogram
+    __shared__ uint32_t rs[BMAX];      // round start of each bucket in the staging area
+    __shared__ uint32_t dbase[BMAX];   // destination of staged slot i is dbase[bucket] + i
+    __shared__ uint32_t wtot[NW];
     __shared__ T stg[SCAT_ROUND];
     __shared__ uint16_t stb[SCAT_ROUND];
     __shared__ int32_t stv[KV ? SCAT_ROUND : 1];
@@ -460,10 +493,13 @@ __global__ __launch_bounds__(SCAT_P) void k_scatter(const PartSeg* ps, const Chu
     const ChunkRef c = cr[blockIdx.x];
     const PartSeg s = ps[c.seg];
     const uint32_t F = 1u << s.logf, B = 2 * F;
-    const uint32_t t = threadIdx.x;
-    const Guide g = load_classifier<K, SCAT_P>(s, trees, guides, lsrt, lguide);
+    const uint32_t t = threadIdx.x, lane = t & 63u, wid = t >> 6;
+    const Guide g = load_classifier<K, SCAT_P>(s, trees, guides, lsrt, lguide, lt);
     const uint32_t* h = hist + s.hist_base + (uint64_t)c.idx * B;
-    for (uint32_t j = t; j < B; j += SCAT_P) cursor[j] = (uint32_t)bstart[s.bkt_base + j] + h[j];
+    for (uint32_t j = t; j < BMAX; j += SCAT_P) {
+        cursor[j] = j < B ? (uint32_t)bstart[s.bkt_base + j] + h[j] : 0u;
+        rh[j] = 0;
+    }
     const T* src = (s.buf ? b : a) + s.begin;
     T* dst = (s.buf ? a : b) + s.begin;
     const int32_t* vsrc = KV ? ((s.buf ? vb : va) + s.begin) : nullptr;
@@ -471,39 +507,56 @@ __global__ __launch_bounds__(SCAT_P) void k_scatter(const PartSeg* ps, const Chu
     const uint64_t lo = (uint64_t)c.idx * s.chunk;
     const uint64_t hi = lo + s.chunk < s.size ? lo + s.chunk : s.size;
 
-    for (uint64_t base = lo; base < hi; base += SCAT_ROUND) {
+    T x[SCAT_E], xn[SCAT_E];
+    int32_t xv[SCAT_E], xvn[SCAT_E];
+    auto load_round = [&](uint64_t base, T (&dx)[SCAT_E], int32_t (&dv)[SCAT_E]) {
         const uint32_t cnt = (uint32_t)((hi - base) < (uint64_t)SCAT_ROUND ? (hi - base) : (uint64_t)SCAT_ROUND);
-        for (uint32_t j = t; j < B; j += SCAT_P) rh[j] = 0;
-        __syncthreads();
-        T x[SCAT_E];
-        int32_t xv[SCAT_E];
-        uint32_t bk[SCAT_E], lr[SCAT_E];
 #pragma unroll
         for (int e = 0; e < SCAT_E; ++e) {
             const uint32_t i = e * SCAT_P + t;
             const uint64_t ic = base + (i < cnt ? i : cnt - 1);  // in bounds, unconditional
-            x[e] = src[ic];
-            if constexpr (KV) xv[e] = vsrc[ic];
+            dx[e] = src[ic];
+            if constexpr (KV) dv[e] = vsrc[ic];
         }
+    };
+    if (lo < hi) load_round(lo, x, xv);
+    __syncthreads();
+
+    for (uint64_t base = lo; base < hi; base += SCAT_ROUND) {
+        const uint32_t cnt = (uint32_t)((hi - base) < (uint64_t)SCAT_ROUND ? (hi - base) : (uint64_t)SCAT_ROUND);
+        if (base + SCAT_ROUND < hi) load_round(base + SCAT_ROUND, xn, xvn);  // prefetch
+        uint32_t bk[SCAT_E], lr[SCAT_E];
 #pragma unroll
         for (int e = 0; e < SCAT_E; ++e) {
             const uint32_t i = e * SCAT_P + t;
-            if (i < cnt) {
-                bk[e] = classify(lsrt, lguide, g, Tr::enc(x[e]));
-                lr[e] = atomicAdd(&rh[bk[e]], 1u);
-            }
+            bk[e] = classify_seg(s, lsrt, lguide, g, lt, Tr::enc(x[e]));
+            if (i < cnt) lr[e] = atomicAdd(&rh[bk[e]], 1u);
         }
-        __syncthreads();
-        // exclusive scan of the round histogram: two buckets per thread
-        {
-            const uint32_t j0 = 2 * t, j1 = 2 * t + 1;
-            const uint32_t h0 = j0 < B ? rh[j0] : 0u, h1 = j1 < B ? rh[j1] : 0u;
-            uint32_t tot;
-            const uint32_t p = block_excl_scan<SCAT_P>(h0 + h1, wtot, &tot);
-            if (j0 < B) rs[j0] = p;
-            if (j1 < B) rs[j1] = p + h0;
+        __syncthreads();  // 1
+        // exclusive scan of rh: thread t owns buckets 2t, 2t+1
+        const uint32_t j0 = 2 * t, j1 = 2 * t + 1;
+        const uint32_t h0 = rh[j0], h1 = rh[j1];
+        uint32_t v = h0 + h1, incl = v;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t y = __shfl_up(incl, d);
+            if (lane >= (uint32_t)d) incl += y;
         }
-        __syncthreads();
+        if (lane == 63) wtot[wid] = incl;
+        __syncthreads();  // 2
+        uint32_t woff = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) woff += (w < (int)wid) ? wtot[w] : 0u;
+        const uint32_t p0 = woff + incl - v, p1 = p0 + h0;
+        rs[j0] = p0;
+        rs[j1] = p1;
+        dbase[j0] = cursor[j0] - p0;
+        dbase[j1] = cursor[j1] - p1;
+        cursor[j0] += h0;
+        cursor[j1] += h1;
+        rh[j0] = 0;
+        rh[j1] = 0;
+        __syncthreads();  // 3
 #pragma unroll
         for (int e = 0; e < SCAT_E; ++e) {
             const uint32_t i = e * SCAT_P + t;
@@ -514,12 +567,12 @@ __global__ __launch_bounds__(SCAT_P) void k_scatter(const PartSeg* ps, const Chu
                 if constexpr (KV) stv[p] = xv[e];
             }
         }
-        __syncthreads();
+        __syncthreads();  // 4
 #pragma unroll
         for (int e = 0; e < SCAT_E; ++e) {
             const uint32_t i = e * SCAT_P + t;  // staged keys are grouped by bucket
             const uint32_t bkt = stb[i] & (BMAX - 1u);  // slots >= cnt: stale, never stored
-            const uint64_t d = (uint64_t)cursor[bkt] + (i - rs[bkt]);
+            const uint64_t d = (uint64_t)dbase[bkt] + i;
             const T y = stg[i];
             int32_t yv = 0;
             if constexpr (KV) yv = stv[i];
@@ -528,9 +581,14 @@ __global__ __launch_bounds__(SCAT_P) void k_scatter(const PartSeg* ps, const Chu
                 if constexpr (KV) vdst[d] = yv;
             }
         }
-        __syncthreads();
-        for (uint32_t j = t; j < B; j += SCAT_P) cursor[j] += rh[j];
-        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < SCAT_E; ++e) {
+            x[e] = xn[e];
+            if constexpr (KV) xv[e] = xvn[e];
+        }
+        // the next round's atomics touch rh (cleared before barrier 3) and the
+        // staging area is rewritten only after its barrier 3, which every
+        // thread reaches after finishing the stores above
     }
 }
 
@@ -604,7 +662,10 @@ struct PlanLists {
             pl.copy[o + p] = TileSeg{begin + pb, (uint32_t)((sz - pb) < COPY_PIECE ? (sz - pb) : COPY_PIECE), 1u};
         }
     } else {
-        pl.next[o] = Seg{begin, sz, buf, 0u};
+        // a bucket of a learned level with a finer level below keeps the
+        // learned classifier: aux = (bucket index << 1) | 1
+        const uint32_t aux = (s.mode && s.lshift) ? ((((b >> 1) + s.lbase) << 1) | 1u) : 0u;
+        pl.next[o] = Seg{begin, sz, buf, aux};
     }
 }
 

@@ -1,0 +1,23 @@
+// s512_learned_host.hpp -- host side of the learned partitioner.
+#pragma once
+#include "s512_learned.hpp"
+#include "s512_runtime.hpp"
+#include "s512_engine.hpp"
+
+namespace s512 {
+
+struct LearnedModel {
+    LearnedTable table;   // device table + feature transform
+    const float* params;  // device parameters after training
+    float loss;           // final full-batch training loss
+    bool degenerate;      // all sampled keys equal: no model
+};
+
+// Samples ~prm.sample_permille/1000 of a (device array of n keys), sorts the
+// sample with the engine, trains the model and tabulates it.  Caller holds
+// ds.mu.  Synchronises the stream.
+template <class T>
+s512_status build_learned_model(const T* a, size_t n, hipStream_t st, DeviceState& ds, const EngineParams& prm,
+                                LearnedModel* out, s512_stats* stats);
+
+}  // namespace s512

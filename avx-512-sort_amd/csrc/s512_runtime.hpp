@@ -81,6 +81,7 @@ struct DeviceState {
     DevBuf data;      // second key array (and value array for key/value sorts)
     DevBuf arena;     // per-level metadata, histograms, lists
     DevBuf stage;     // caller-host-pointer staging
+    DevBuf learn;     // learned partitioner: sample, training state, table
     HostBuf pinned;   // host<->device metadata staging
     HostBuf pinned2;
     std::vector<hipEvent_t> events;  // profiling event pool

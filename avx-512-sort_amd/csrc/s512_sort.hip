@@ -14,7 +14,9 @@
 #include "s512_runtime.hpp"
 #include "s512_engine.hpp"
 #include "s512_tile_f64.hpp"
+#include "s512_learned_host.hpp"
 #include <algorithm>
+#include <type_traits>
 
 namespace s512 {
 
@@ -81,7 +83,7 @@ template <class T, bool KV>
 static void launch_tile(int cls, const TileSeg* segs, const uint32_t* dcount, uint32_t count, T* a, int32_t* va,
                         const T* b, const int32_t* vb, hipStream_t st, bool f64_fast = false) {
     if constexpr (KV) launch_tile_kv(cls, segs, dcount, count, a, va, b, vb, st);
-    else if constexpr (sizeof(T) == 8) {
+    else if constexpr (std::is_same<T, double>::value) {
         if (f64_fast) launch_tile_f64_fast(cls, segs, dcount, count, a, b, st);
         else launch_tile_keys<T>(cls, segs, dcount, count, a, b, st);
     } else launch_tile_keys<T>(cls, segs, dcount, count, a, b, st);
@@ -97,8 +99,6 @@ static inline int ceil_log2(uint64_t x) {
 template <class T, bool KV>
 s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const EngineParams& prm, s512_stats* stats,
                         bool profile) {
-    using K = typename KeyTraits<T>::K;
-    using TC = TileCfg<T, KV>;
     if (stats) memset(stats, 0, sizeof(*stats));
     if (n <= 1) return S512_OK;
     if (n >= (1ull << 32)) {
@@ -110,6 +110,16 @@ s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const Engin
     if (rc) return rc;
     DeviceState& ds = device_state(dev);
     std::lock_guard<std::mutex> lk(ds.mu);
+    return sort_engine_locked<T, KV>(a, va, n, st, prm, stats, profile, ds);
+}
+
+template <class T, bool KV>
+s512_status sort_engine_locked(T* a, int32_t* va, size_t n, hipStream_t st, const EngineParams& prm,
+                               s512_stats* stats, bool profile, DeviceState& ds) {
+    using K = typename KeyTraits<T>::K;
+    using TC = TileCfg<T, KV>;
+    if (n <= 1) return S512_OK;
+    s512_status rc;
     Profiler pf;
     pf.on = profile && stats;
     pf.st = st;
@@ -144,13 +154,32 @@ s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const Engin
         return S512_OK;
     }
 
+    // ---- learned partitioner: model of the key CDF for the first two levels
+    LearnedModel model;
+    memset(&model, 0, sizeof(model));
+    bool use_learned = false;
+    uint32_t lf1 = 0, lf2 = 0;
+    if constexpr (!KV && (std::is_same<T, double>::value || std::is_same<T, int32_t>::value)) {
+        if (prm.learned) {
+            if ((rc = build_learned_model<T>(a, n, st, ds, prm, &model, stats))) return rc;
+            use_learned = !model.degenerate;
+            const uint64_t leaves = (n + prm.target_leaf - 1) / prm.target_leaf;
+            const int total = std::max(1, ceil_log2(leaves));
+            lf1 = (uint32_t)std::min(total, prm.max_logf);
+            lf2 = (uint32_t)std::max(0, std::min(total - (int)lf1, prm.max_logf));
+            model.table.nb_log = lf1 + lf2;
+        }
+    }
+    bool learned_checked = false;
+
     // ---- scratch arrays (same size as the input)
     const size_t dbytes = n * sizeof(T) + (KV ? n * sizeof(int32_t) : 0);
     if ((rc = ds.data.reserve(dbytes, st))) return rc;
     T* b = (T*)ds.data.ptr;
     int32_t* vb = KV ? (int32_t*)((char*)ds.data.ptr + n * sizeof(T)) : nullptr;
 
-    std::vector<Seg> segs{Seg{0, n, 0, 0}};
+    // aux = 1: the segment is partitioned by the learned model (first level)
+    std::vector<Seg> segs{Seg{0, n, 0, use_learned ? 1u : 0u}};
     std::vector<PartSeg> ps;
     std::vector<ChunkRef> crs;
     const uint32_t smax = 8192;
@@ -179,6 +208,23 @@ s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const Engin
             p.buf = segs[i].buf;
             int lf = ceil_log2((p.size + prm.target_leaf - 1) / prm.target_leaf);
             lf = std::max(1, std::min(lf, prm.max_logf));
+            p.mode = 0;
+            p.lshift = 0;
+            p.lbase = 0;
+            if (use_learned && (segs[i].aux & 1u)) {
+                // learned: level 1 uses the top lf1 bits of the fine bucket, level 2 the low lf2 bits
+                p.mode = 1;
+                if (level == 1) {
+                    lf = (int)lf1;
+                    p.lshift = lf2;
+                } else if (p.size > 2 * (n >> lf1)) {
+                    // the model under-split this range: sampled splitters below it
+                    p.mode = 0;
+                } else {
+                    lf = (int)std::max<uint32_t>(lf2, 1);
+                    p.lbase = (segs[i].aux >> 1) << lf2;
+                }
+            }
             p.logf = (uint32_t)lf;
             const uint32_t F = 1u << lf, B = 2 * F;
             p.nsample = std::min<uint32_t>(smax, prm.oversample * F);
@@ -190,7 +236,7 @@ s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const Engin
             p.guide_base = sumG;
             p.kmin = 0;
             p.shift = 0;
-            p.pad = 0;
+            if (p.mode) p.nsample = 0;
             sumF += F;
             sumG += (uint64_t)GUIDE_PER_SPLIT * F;
             histN += (uint64_t)p.nchunks * B;
@@ -254,35 +300,64 @@ s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const Engin
         uint64_t* h_cnt64 = hp.take<uint64_t>(16);
         memcpy(h_ps, ps.data(), m * sizeof(PartSeg));
         memcpy(h_cr, crs.data(), tc * sizeof(ChunkRef));
-        for (uint32_t i = 0; i < m; ++i) h_sl[i] = TileSeg{(uint64_t)i * smax, ps[i].nsample, 0};
+        uint32_t nsplit = 0;  // sample lists of the segments partitioned by splitters
+        for (uint32_t i = 0; i < m; ++i)
+            if (!ps[i].mode) h_sl[nsplit++] = TileSeg{(uint64_t)i * smax, ps[i].nsample, 0};
         memset(h_cnt, 0, 32 * sizeof(uint32_t));
         memset(h_cnt64, 0, 16 * sizeof(uint64_t));
-        h_cnt[10] = m;
+        h_cnt[10] = nsplit;
         S512_HIP(hipMemcpyAsync(ds.arena.ptr, ds.pinned.ptr, up_bytes, hipMemcpyHostToDevice, st));
 
         const uint64_t seed = prm.seed + 0x9E37ull * (uint64_t)level;
+        bool any_split = false, any_learned = false;
+        for (auto& p : ps) (p.mode ? any_learned : any_split) = true;
+        if (any_split) {
         pf.begin(S512_PHASE_SAMPLE);
         k_sample<T><<<m, 256, 0, st>>>(d_ps, a, b, d_smp, smax, seed);
         S512_CHECK_LAUNCH();
         pf.end();
         pf.begin(S512_PHASE_SAMPLE_SORT);
-        k_tile_sort<K, 512, 16><<<m, 512, 0, st>>>(d_sl, d_cnt + 10, d_smp, d_smp);
+        k_tile_sort<K, 512, 16><<<nsplit, 512, 0, st>>>(d_sl, d_cnt + 10, d_smp, d_smp);
         S512_CHECK_LAUNCH();
         pf.end();
         pf.begin(S512_PHASE_TREE);
         k_build_tree<K><<<m, FMAX, 0, st>>>(d_ps, d_smp, smax, d_tree, d_guide);
         S512_CHECK_LAUNCH();
         pf.end();
+        }
         pf.begin(S512_PHASE_COUNT);
-        k_count<T><<<tc, CNT_P, 0, st>>>(d_ps, d_cr, a, b, d_tree, d_guide, d_hist, d_cnt + 11);
+        k_count<T><<<tc, CNT_P, 0, st>>>(d_ps, d_cr, a, b, d_tree, d_guide, d_hist, d_cnt + 11, model.table);
         S512_CHECK_LAUNCH();
         pf.end();
         pf.begin(S512_PHASE_SCAN);
         k_colscan<<<m, BMAX, 0, st>>>(d_ps, d_hist, d_bstart, d_bsize);
         S512_CHECK_LAUNCH();
         pf.end();
+        if (any_learned && level == 1 && !learned_checked) {
+            // learned level 1: fall back to sampled splitters when the model
+            // leaves a bucket far above the mean (nothing has moved yet)
+            learned_checked = true;
+            const uint32_t B1 = 2u << lf1;
+            if ((rc = ds.pinned2.reserve(B1 * sizeof(uint64_t), st))) return rc;
+            uint64_t* hb = (uint64_t*)ds.pinned2.ptr;
+            S512_HIP(hipMemcpyAsync(hb, d_bsize, B1 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+            S512_HIP(hipStreamSynchronize(st));
+            uint64_t mx = 0;
+            for (uint32_t j = 0; j < B1; ++j) mx = std::max(mx, hb[j]);
+            const double mean = (double)n / (double)(1u << lf1);
+            if (stats) stats->learned_max_bucket_ratio = (float)((double)mx / mean);
+            if ((double)mx > prm.imbalance * mean) {
+                if (stats) stats->learned_fallback = 1;
+                use_learned = false;
+                segs.assign(1, Seg{0, n, 0, 0});
+                level = 0;
+                continue;
+            }
+            if (stats) stats->learned_levels = lf2 ? 2 : 1;
+        }
         pf.begin(S512_PHASE_SCATTER);
-        k_scatter<T, KV><<<tc, SCAT_P, 0, st>>>(d_ps, d_cr, a, b, va, vb, d_tree, d_guide, d_hist, d_bstart);
+        k_scatter<T, KV><<<tc, SCAT_P, 0, st>>>(d_ps, d_cr, a, b, va, vb, d_tree, d_guide, d_hist, d_bstart,
+                                                model.table);
         S512_CHECK_LAUNCH();
         pf.end();
         pf.begin(S512_PHASE_PLAN);
@@ -299,7 +374,7 @@ s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const Engin
         memcpy(counts, h_out, sizeof(counts));
         memcpy(counts64, (char*)h_out + ((char*)d_cnt64 - (char*)d_cnt), sizeof(counts64));
         // level 1 saw every key: fp64 leaves may use min/max when no key is NaN or -0.0
-        if (level == 1) f64_fast = !KV && sizeof(T) == 8 && prm.f64_minmax && counts[11] == 0;
+        if (level == 1) f64_fast = !KV && std::is_same<T, double>::value && prm.f64_minmax && counts[11] == 0;
         if (counts[8]) {
             pf.begin(S512_PHASE_COPY);
             k_copy_back<T, KV><<<std::min<uint32_t>(counts[8], 16384), 256, 0, st>>>(pl.copy, pl.copy_count, a, b, va,
@@ -333,6 +408,7 @@ s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const Engin
     S512_HIP(hipStreamSynchronize(st));
     if (stats) {
         stats->levels = level;
+        stats->classifier = use_learned ? 1 : 0;
         stats->tiles = tiles;
         memcpy(stats->keys, keys_ph, sizeof(keys_ph));
         pf.collect(stats->kernel_ms, stats->launches);
@@ -341,6 +417,8 @@ s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const Engin
 }
 
 template s512_status sort_engine<int32_t, false>(int32_t*, int32_t*, size_t, hipStream_t, const EngineParams&, s512_stats*, bool);
+template s512_status sort_engine_locked<uint64_t, false>(uint64_t*, int32_t*, size_t, hipStream_t, const EngineParams&, s512_stats*, bool, DeviceState&);
+template s512_status sort_engine_locked<uint32_t, false>(uint32_t*, int32_t*, size_t, hipStream_t, const EngineParams&, s512_stats*, bool, DeviceState&);
 template s512_status sort_engine<double, false>(double*, int32_t*, size_t, hipStream_t, const EngineParams&, s512_stats*, bool);
 template s512_status sort_engine<int32_t, true>(int32_t*, int32_t*, size_t, hipStream_t, const EngineParams&, s512_stats*, bool);
 
@@ -354,6 +432,11 @@ EngineParams default_params(bool wide) {
     p.max_logf = FMAX_LOG;
     p.seed = 0x5EEDull;
     p.f64_minmax = true;
+    p.learned = false;
+    p.sample_permille = 10;
+    p.train_steps = 100;
+    p.learn_rate = 2e-3f;
+    p.imbalance = 4.0f;
     return p;
 }
 
@@ -364,6 +447,10 @@ EngineParams params_from(const s512_options* opt, bool wide) {
     if (opt->oversample) p.oversample = opt->oversample;
     if (opt->max_fanout_log2) p.max_logf = std::max(1, std::min<int>((int)opt->max_fanout_log2, FMAX_LOG));
     if (opt->seed) p.seed = opt->seed;
+    p.learned = opt->classifier == S512_CLASSIFY_LEARNED;
+    if (opt->sample_permille) p.sample_permille = opt->sample_permille;
+    if (opt->train_steps) p.train_steps = opt->train_steps;
+    if (opt->imbalance_pct) p.imbalance = (float)opt->imbalance_pct / 100.0f;
     return p;
 }
 
@@ -404,6 +491,14 @@ using namespace s512;
 static s512_status check_opts(const s512_options* opt) {
     if (opt && opt->oversample > 512) {
         set_error("oversample %u too large (max 512)", opt->oversample);
+        return S512_ERR_INVALID;
+    }
+    if (opt && opt->classifier > S512_CLASSIFY_LEARNED) {
+        set_error("unknown classifier %u", opt->classifier);
+        return S512_ERR_INVALID;
+    }
+    if (opt && (opt->sample_permille > 1000 || opt->train_steps > 100000)) {
+        set_error("sample_permille (<= 1000) or train_steps (<= 100000) out of range");
         return S512_ERR_INVALID;
     }
     if (opt && opt->target_leaf && opt->target_leaf < 64) {

@@ -32,6 +32,35 @@ __global__ void k_fill_f64(double* out, uint64_t n, int kind, uint64_t seed, uin
             case S512X_DISTINCT: v = (double)(sm64(seed, i) % (param ? param : 1)); break;
             case S512X_SORTED: v = (double)i / (double)n; break;
             case S512X_REVERSED: v = (double)(n - 1 - i) / (double)n; break;
+            case S512X_MIXTURE: {
+                // 90%: one of 8 Gaussian components (equal weights), means in
+                // [-20, 20) and sd in [0.02, 2.02) drawn from the seed; 10%:
+                // lognormal(mu = 0, sigma = 2)
+                const double u0 = u01(seed, 3 * i);
+                double u1 = u01(seed, 3 * i + 1), u2 = u01(seed, 3 * i + 2);
+                if (u1 < 0x1.0p-53) u1 = 0x1.0p-53;
+                const double z = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+                if (u0 < 0.1) {
+                    v = exp(2.0 * z);
+                } else {
+                    const uint64_t k = (uint64_t)((u0 - 0.1) / 0.9 * 8.0) & 7u;
+                    const double mean = -20.0 + 40.0 * u01(seed ^ 0xA5A5ull, 1000 + k);
+                    const double su = u01(seed ^ 0xA5A5ull, 2000 + k);
+                    v = mean + (0.02 + 2.0 * su * su) * z;
+                }
+                break;
+            }
+            case S512X_PIECEWISE: {
+                // param pieces (default 64) of equal mass, widths alternating
+                // 1 and 1e-9: piece p starts after ceil(p/2) wide and floor(p/2)
+                // narrow pieces; uniform inside the piece
+                const uint64_t K = param ? param : 64;
+                const uint64_t pc = sm64(seed, 2 * i) % K;
+                const double w = (pc & 1) ? 1e-9 : 1.0;
+                const double x0 = (double)((pc + 1) / 2) + 1e-9 * (double)(pc / 2);
+                v = x0 + w * u01(seed, 2 * i + 1);
+                break;
+            }
             default: v = 0.0; break;
         }
         out[i] = v;
@@ -64,7 +93,10 @@ static unsigned grid_for(size_t n) {
 
 extern "C" s512_status s512x_fill_f64(double* d, size_t n, int kind, uint64_t seed, uint64_t param, void* stream) {
     if (!d && n) { set_error("null array"); return S512_ERR_INVALID; }
-    if (kind < S512X_UNIFORM || kind > S512X_REVERSED) { set_error("unknown f64 distribution %d", kind); return S512_ERR_INVALID; }
+    if (kind < S512X_UNIFORM || kind > S512X_PIECEWISE || kind == S512X_INDEX) {
+        set_error("unknown f64 distribution %d", kind);
+        return S512_ERR_INVALID;
+    }
     if (!n) return S512_OK;
     k_fill_f64<<<grid_for(n), 256, 0, (hipStream_t)stream>>>(d, n, kind, seed, param);
     S512_CHECK_LAUNCH();
@@ -73,7 +105,10 @@ extern "C" s512_status s512x_fill_f64(double* d, size_t n, int kind, uint64_t se
 
 extern "C" s512_status s512x_fill_i32(int32_t* d, size_t n, int kind, uint64_t seed, uint64_t param, void* stream) {
     if (!d && n) { set_error("null array"); return S512_ERR_INVALID; }
-    if (kind == S512X_NORMAL || kind < S512X_UNIFORM || kind > S512X_INDEX) { set_error("unknown i32 distribution %d", kind); return S512_ERR_INVALID; }
+    if (kind == S512X_NORMAL || kind < S512X_UNIFORM || kind > S512X_INDEX) {
+        set_error("unknown i32 distribution %d", kind);
+        return S512_ERR_INVALID;
+    }
     if (!n) return S512_OK;
     k_fill_i32<<<grid_for(n), 256, 0, (hipStream_t)stream>>>(d, n, kind, seed, param);
     S512_CHECK_LAUNCH();

@@ -72,6 +72,8 @@ _SIGS = {
     "s512_merge_inplace_f64": ([_vp, _c_size, _c_size], ctypes.c_int),
     "s512_merge_inplace_i32_dev": ([_vp, _c_size, _c_size, _vp], ctypes.c_int),
     "s512_merge_inplace_f64_dev": ([_vp, _c_size, _c_size, _vp], ctypes.c_int),
+    "s512_mlp_nparams": ([], ctypes.c_uint32),
+    "s512_mlp_loss_grad_f32": ([_vp, _vp, _vp, ctypes.c_uint32, _vp, _vp, _vp], ctypes.c_int),
     "s512x_fill_f64": ([_vp, _c_size, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, _vp], ctypes.c_int),
     "s512x_fill_i32": ([_vp, _c_size, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, _vp], ctypes.c_int),
 }
@@ -88,21 +90,32 @@ PHASES = ["sample", "sample_sort", "tree", "count", "scan", "scatter", "plan", "
 class Options(ctypes.Structure):
     """s512_options (include/s512.h)."""
     _fields_ = [("target_leaf", ctypes.c_uint32), ("oversample", ctypes.c_uint32),
-                ("max_fanout_log2", ctypes.c_uint32), ("profile", ctypes.c_uint32), ("seed", ctypes.c_uint64)]
+                ("max_fanout_log2", ctypes.c_uint32), ("profile", ctypes.c_uint32), ("seed", ctypes.c_uint64),
+                ("classifier", ctypes.c_uint32), ("sample_permille", ctypes.c_uint32),
+                ("train_steps", ctypes.c_uint32), ("imbalance_pct", ctypes.c_uint32)]
+
+
+SPLITTERS, LEARNED = 0, 1
 
 
 class Stats(ctypes.Structure):
     """s512_stats (include/s512.h)."""
     _fields_ = [("levels", ctypes.c_int32), ("tiles", ctypes.c_uint64), ("kernel_ms", ctypes.c_double * NPHASE),
-                ("launches", ctypes.c_uint32 * NPHASE), ("keys", ctypes.c_uint64 * NPHASE)]
+                ("launches", ctypes.c_uint32 * NPHASE), ("keys", ctypes.c_uint64 * NPHASE),
+                ("classifier", ctypes.c_int32), ("learned_fallback", ctypes.c_int32),
+                ("learned_levels", ctypes.c_int32), ("learned_max_bucket_ratio", ctypes.c_float),
+                ("model_loss", ctypes.c_float)]
 
     def as_dict(self):
-        return {"levels": self.levels, "tiles": self.tiles,
+        return {"levels": self.levels, "tiles": self.tiles, "classifier": self.classifier,
+                "learned_fallback": self.learned_fallback, "learned_levels": self.learned_levels,
+                "learned_max_bucket_ratio": self.learned_max_bucket_ratio, "model_loss": self.model_loss,
                 "phases": {PHASES[i]: {"ms": self.kernel_ms[i], "launches": self.launches[i], "keys": self.keys[i]}
                            for i in range(NPHASE) if self.launches[i] or self.keys[i]}}
 
 
-SYNTH = {"uniform": 0, "normal": 1, "distinct": 2, "sorted": 3, "reversed": 4, "index": 5}
+SYNTH = {"uniform": 0, "normal": 1, "distinct": 2, "sorted": 3, "reversed": 4, "index": 5, "mixture": 6,
+         "piecewise": 7}
 
 
 def fill(t, kind, seed=0, param=0):
@@ -140,9 +153,28 @@ def declared_symbols():
     import re
     names = set()
     for h in glob.glob(os.path.join(_HERE, "..", "include", "*.h")):
-        for m in re.finditer(r"^\s*(?:s512_status|const char\*)\s+(s512\w+)\s*\(", open(h).read(), re.M):
+        for m in re.finditer(r"^\s*(?:s512_status|const char\*|uint32_t)\s+(s512\w+)\s*\(", open(h).read(), re.M):
             names.add(m.group(1))
     return names
+
+
+def sort_learned(a, profile=False, **kw):
+    """Sort with the learned CDF partitioner (fallback: sampled splitters)."""
+    opt = Options()
+    opt.classifier = LEARNED
+    for k, v in kw.items():
+        setattr(opt, k, v)
+    return sort_ex(a, opt, profile=profile)
+
+
+def mlp_loss_grad(params, u, t):
+    """GPU fp32 loss and gradient of the CDF model (device float32 tensors)."""
+    import torch
+    g = torch.empty(lib.s512_mlp_nparams(), dtype=torch.float32, device=params.device)
+    loss = torch.empty(1, dtype=torch.float32, device=params.device)
+    _check(lib.s512_mlp_loss_grad_f32(_ptr(params), _ptr(u), _ptr(t), u.numel(), _ptr(g), _ptr(loss),
+                                      _stream_of(params)))
+    return float(loss.item()), g
 
 
 def _check(status):

@@ -117,12 +117,23 @@ enum {
     S512_NPHASE = 10
 };
 
+enum {
+    S512_CLASSIFY_SPLITTERS = 0, /* sampled splitters at every level (default) */
+    S512_CLASSIFY_LEARNED = 1    /* learned CDF model for the first two levels,
+                                    sampled splitters as the fallback        */
+};
+
 typedef struct s512_options {
     uint32_t target_leaf;   /* mean leaf size the fan-out aims for (0: 2048 for 64-bit keys, 4096 for 32-bit) */
     uint32_t oversample;    /* samples per splitter (0: 16)                  */
     uint32_t max_fanout_log2; /* cap on log2 of the per-level fan-out, 1..9 (0: 9) */
     uint32_t profile;       /* nonzero: time every kernel with HIP events    */
     uint64_t seed;          /* sample positions (0: fixed default)           */
+    uint32_t classifier;    /* S512_CLASSIFY_*; learned applies to keys-only int32/double sorts */
+    uint32_t sample_permille; /* learned: model sample, per mille of n (0: 10 = 1%) */
+    uint32_t train_steps;   /* learned: Adam steps (0: 100)                 */
+    uint32_t imbalance_pct; /* learned: fall back to splitters when a first-level bucket
+                               exceeds this percentage of the mean (0: 400) */
 } s512_options;
 
 typedef struct s512_stats {
@@ -131,11 +142,24 @@ typedef struct s512_stats {
     double kernel_ms[S512_NPHASE];      /* device time per phase (profile)   */
     uint32_t launches[S512_NPHASE];     /* kernel launches per phase         */
     uint64_t keys[S512_NPHASE];         /* keys processed per phase          */
+    int32_t classifier;                 /* classifier that produced the result */
+    int32_t learned_fallback;           /* 1: the learned model was rejected  */
+    int32_t learned_levels;             /* levels partitioned by the model    */
+    float learned_max_bucket_ratio;     /* largest first-level bucket / mean  */
+    float model_loss;                   /* final training loss (MSE on CDF)   */
 } s512_stats;
 
 /* Sort512::Sort with options and statistics (device pointers). */
 s512_status s512_sort_i32_dev_ex(int32_t* d_a, size_t n, const s512_options* opt, s512_stats* stats, void* stream);
 s512_status s512_sort_f64_dev_ex(double* d_a, size_t n, const s512_options* opt, s512_stats* stats, void* stream);
+
+/* Learned partitioner model check: full-batch loss and gradient (fp32) of the
+ * monotone 1-32-32-1 CDF model at parameters d_params (s512_mlp_nparams()
+ * floats) over m points (d_u features in [0,1], d_t targets); device pointers.
+ * No reference counterpart (the reference has no learned path).            */
+uint32_t s512_mlp_nparams(void);
+s512_status s512_mlp_loss_grad_f32(const float* d_params, const float* d_u, const float* d_t, uint32_t m,
+                                   float* d_grad, float* d_loss, void* stream);
 
 /* -------------------------------------------------------------- key/value */
 /* Sort512kv::Sort<int,size_t>    sort512kv.hpp:6132-6135.  Keys ascending;

@@ -17,7 +17,10 @@ enum {
     S512X_DISTINCT = 2,  /* SplitMix64(seed,i) % param                                 */
     S512X_SORTED = 3,    /* f64: i/n;  i32: i - n/2                                    */
     S512X_REVERSED = 4,  /* f64: (n-1-i)/n;  i32: n/2 - i                              */
-    S512X_INDEX = 5      /* i32 only: i                                                */
+    S512X_INDEX = 5,     /* i32 only: i                                                */
+    S512X_MIXTURE = 6,   /* f64 only: 8-component Gaussian mixture + 10% lognormal(0,2) */
+    S512X_PIECEWISE = 7  /* f64 only: piecewise-constant density, param pieces of equal
+                            mass with widths alternating 1 and 1e-9 (default 64)     */
 };
 
 s512_status s512x_fill_f64(double* d_out, size_t n, int kind, uint64_t seed, uint64_t param, void* stream);

@@ -31,7 +31,8 @@
  *   - Synthetic input generators (SplitMix64 counter PRNG, Box-Muller,
  *     POSIX drand48 as used by the reference tests, sort512test.cpp:77-82).
  *   - fp64 forward/backward of the 1-32-32-1 monotone MLP CDF model used by
- *     the learned partitioner (no reference counterpart).
+ *     the learned partitioner (no reference counterpart; clamped-ramp
+ *     activation, squared weights).
  *
  * Build: oracle/Makefile -> oracle/libs512oracle.so (gcc -O2, no AVX needed).
  */
@@ -411,25 +412,30 @@ void orc_rand_vec_f64(double *a, size_t n)
 
 /* ------------------------------------------------------------------------ */
 /* Learned partitioner model, fp64 restatement (no reference counterpart).   */
-/* Monotone 1-H-H-1 MLP:  h1 = relu(w1*x + b1), h2 = relu(W2 h1 + b2),       */
-/* y = w3 . h2 + b3, with w1, W2, w3 >= 0 obtained as squares of the free    */
-/* parameters (w = p*p), so y is non-decreasing in x.                        */
+/* Monotone 1-H-H-1 MLP with the clamped ramp r(z) = min(max(z,0),1):        */
+/*   h1 = r(w1*x + b1), h2 = r(W2 h1 + b2), y = w3 . h2 + b3                 */
+/* with w1, W2, w3 >= 0 obtained as squares of the free parameters (w = p*p),*/
+/* so y is non-decreasing in x; the bounded ramp (unlike a plain ReLU, which */
+/* with non-negative weights only yields convex functions) lets the net fit  */
+/* S-shaped CDFs.  r'(z) is taken as 1 on (0,1) and 0 elsewhere.            */
 /* Parameter vector layout (H = hidden width):                               */
 /*   p1[H], b1[H], p2[H*H] (row j = unit j of layer 2), b2[H], p3[H], b3[1]  */
 /* Loss over a batch: mean over i of (y(x_i) - t_i)^2.                       */
 /* ------------------------------------------------------------------------ */
+static double ramp(double z) { return z <= 0 ? 0 : (z >= 1 ? 1 : z); }
+static double dramp(double z) { return (z > 0 && z < 1) ? 1.0 : 0.0; }
+
 size_t orc_mlp_nparams(int H) { return (size_t)H * 3 + (size_t)H * H + (size_t)H + 1; }
 
 double orc_mlp_forward(const double *p, int H, double x)
 {
     const double *p1 = p, *b1 = p + H, *p2 = p + 2 * H, *b2 = p2 + H * H, *p3 = b2 + H, *b3 = p3 + H;
     double h1[256], y = b3[0];
-    for (int j = 0; j < H; ++j) { double z = p1[j] * p1[j] * x + b1[j]; h1[j] = z > 0 ? z : 0; }
+    for (int j = 0; j < H; ++j) h1[j] = ramp(p1[j] * p1[j] * x + b1[j]);
     for (int j = 0; j < H; ++j) {
         double z = b2[j];
         for (int i = 0; i < H; ++i) z += p2[j * H + i] * p2[j * H + i] * h1[i];
-        double h = z > 0 ? z : 0;
-        y += p3[j] * p3[j] * h;
+        y += p3[j] * p3[j] * ramp(z);
     }
     return y;
 }
@@ -445,12 +451,12 @@ double orc_mlp_loss_grad(const double *p, int H, const double *x, const double *
     double z1[256], h1[256], z2[256], h2[256], d2[256], d1[256];
     for (size_t s = 0; s < n; ++s) {
         double xv = x[s];
-        for (int j = 0; j < H; ++j) { z1[j] = p1[j] * p1[j] * xv + b1[j]; h1[j] = z1[j] > 0 ? z1[j] : 0; }
+        for (int j = 0; j < H; ++j) { z1[j] = p1[j] * p1[j] * xv + b1[j]; h1[j] = ramp(z1[j]); }
         double y = b3[0];
         for (int j = 0; j < H; ++j) {
             double z = b2[j];
             for (int i = 0; i < H; ++i) z += p2[j * H + i] * p2[j * H + i] * h1[i];
-            z2[j] = z; h2[j] = z > 0 ? z : 0;
+            z2[j] = z; h2[j] = ramp(z);
             y += p3[j] * p3[j] * h2[j];
         }
         double e = y - t[s];
@@ -459,7 +465,7 @@ double orc_mlp_loss_grad(const double *p, int H, const double *x, const double *
         gb3[0] += dy;
         for (int j = 0; j < H; ++j) {
             g3[j] += dy * 2.0 * p3[j] * h2[j];
-            d2[j] = z2[j] > 0 ? dy * p3[j] * p3[j] : 0.0;
+            d2[j] = dy * p3[j] * p3[j] * dramp(z2[j]);
             gb2[j] += d2[j];
         }
         for (int i = 0; i < H; ++i) d1[i] = 0;
@@ -472,7 +478,7 @@ double orc_mlp_loss_grad(const double *p, int H, const double *x, const double *
             }
         }
         for (int i = 0; i < H; ++i) {
-            double d = z1[i] > 0 ? d1[i] : 0.0;
+            double d = d1[i] * dramp(z1[i]);
             gb1[i] += d;
             g1[i] += d * 2.0 * p1[i] * xv;
         }
