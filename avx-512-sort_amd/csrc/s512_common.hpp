@@ -75,6 +75,8 @@ struct PartSeg {
     uint32_t mode;       // 0: sampled splitters, 1: learned model
     uint32_t lshift;     // learned: bucket = (fine bucket >> lshift) - lbase
     uint32_t lbase;
+    uint32_t pad;
+    uint64_t sample_base;  // offset of this segment's samples in the sample buffer
 };
 
 struct ChunkRef {

@@ -316,17 +316,22 @@ __global__ __launch_bounds__(256) void k_wave_sort_kv(const TileSeg* segs, const
 }
 
 // ---------------------------------------------------------------- sampling
+// Kernels over segments or chunks take the number of work items from device
+// memory and loop over them, so a fixed grid (hipGraph capture, no host
+// round trip) and an exact grid (one item per workgroup) both work.
 template <class T>
-__global__ __launch_bounds__(256) void k_sample(const PartSeg* ps, const T* a, const T* b,
+__global__ __launch_bounds__(256) void k_sample(const PartSeg* ps, const uint32_t* nseg, const T* a, const T* b,
                                                 typename KeyTraits<T>::K* samples, uint32_t smax, uint64_t seed) {
     using Tr = KeyTraits<T>;
-    const PartSeg s = ps[blockIdx.x];
-    const T* src = (s.buf ? b : a) + s.begin;
-    typename Tr::K* out = samples + (uint64_t)blockIdx.x * smax;
-    for (uint32_t i = threadIdx.x; i < s.nsample; i += blockDim.x) {
-        const uint64_t h = mix64(seed ^ (s.begin * 0x632BE59BD9B4E019ull), i);
-        const uint64_t pos = __umul64hi(h, s.size);  // uniform in [0, size)
-        out[i] = Tr::enc(src[pos]);
+    for (uint32_t w = blockIdx.x; w < *nseg; w += gridDim.x) {
+        const PartSeg s = ps[w];
+        const T* src = (s.buf ? b : a) + s.begin;
+        typename Tr::K* out = samples + s.sample_base;
+        for (uint32_t i = threadIdx.x; i < s.nsample; i += blockDim.x) {
+            const uint64_t h = mix64(seed ^ (s.begin * 0x632BE59BD9B4E019ull), i);
+            const uint64_t pos = __umul64hi(h, s.size);  // uniform in [0, size)
+            out[i] = Tr::enc(src[pos]);
+        }
     }
 }
 
@@ -335,15 +340,10 @@ __global__ __launch_bounds__(256) void k_sample(const PartSeg* ps, const T* a, c
 // the sample.  One workgroup per segment.  Writes kmin/shift into the
 // segment descriptor for the count and scatter kernels.
 template <class K>
-__global__ __launch_bounds__(FMAX) void k_build_tree(PartSeg* ps, const K* samples, uint32_t smax, K* trees,
-                                                     uint32_t* guides) {
-    __shared__ K srt[FMAX];
-    __shared__ uint32_t wtot[FMAX / 64];
-    PartSeg& s = ps[blockIdx.x];
-    if (s.mode || s.nsample == 0) return;  // learned segment: no splitters
+__device__ __forceinline__ void build_tree_body(PartSeg& s, const K* smp, K* trees, uint32_t* guides, K* srt,
+                                                uint32_t* wtot) {
     const uint32_t F = 1u << s.logf, G = GUIDE_PER_SPLIT * F;
     const uint32_t ns = s.nsample;
-    const K* smp = samples + (uint64_t)blockIdx.x * smax;
     const uint32_t t = threadIdx.x;
     K c = (K)~(K)0;
     bool keep = false;
@@ -384,6 +384,19 @@ __global__ __launch_bounds__(FMAX) void k_build_tree(PartSeg* ps, const K* sampl
     }
 }
 
+template <class K>
+__global__ __launch_bounds__(FMAX) void k_build_tree(PartSeg* ps, const uint32_t* nseg, const K* samples,
+                                                     uint32_t smax, K* trees, uint32_t* guides) {
+    __shared__ K srt[FMAX];
+    __shared__ uint32_t wtot[FMAX / 64];
+    for (uint32_t w = blockIdx.x; w < *nseg; w += gridDim.x) {
+        PartSeg& s = ps[w];
+        if (s.mode || s.nsample == 0) continue;  // learned segment: no splitters
+        build_tree_body<K>(s, samples + s.sample_base, trees, guides, srt, wtot);
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------- count
 // Encoded fp64 keys that are NaN or -0.0 (they rule out the min/max leaf sort).
 __device__ __forceinline__ bool nan_or_negzero(uint64_t k) {
@@ -392,15 +405,17 @@ __device__ __forceinline__ bool nan_or_negzero(uint64_t k) {
 __device__ __forceinline__ bool nan_or_negzero(uint32_t) { return false; }
 
 template <class T>
-__global__ __launch_bounds__(CNT_P) void k_count(const PartSeg* ps, const ChunkRef* cr, const T* a, const T* b,
-                                                 const typename KeyTraits<T>::K* trees, const uint32_t* guides,
-                                                 uint32_t* hist, uint32_t* special, LearnedTable lt) {
+__global__ __launch_bounds__(CNT_P) void k_count(const PartSeg* ps, const ChunkRef* cr, const uint32_t* nchunk,
+                                                 const T* a, const T* b, const typename KeyTraits<T>::K* trees,
+                                                 const uint32_t* guides, uint32_t* hist, uint32_t* special,
+                                                 LearnedTable lt) {
     using Tr = KeyTraits<T>;
     using K = typename Tr::K;
     __shared__ K lsrt[FMAX];
     __shared__ uint32_t lguide[LDS_TABLE_WORDS];
     __shared__ uint32_t lh[BMAX];
-    const ChunkRef c = cr[blockIdx.x];
+    for (uint32_t w = blockIdx.x; w < *nchunk; w += gridDim.x) {
+    const ChunkRef c = cr[w];
     const PartSeg s = ps[c.seg];
     const uint32_t F = 1u << s.logf, B = 2 * F;
     const uint32_t t = threadIdx.x;
@@ -433,14 +448,19 @@ __global__ __launch_bounds__(CNT_P) void k_count(const PartSeg* ps, const ChunkR
     __syncthreads();
     uint32_t* h = hist + s.hist_base + (uint64_t)c.idx * B;
     for (uint32_t j = t; j < B; j += CNT_P) h[j] = lh[j];
+    __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------- column scan
 // hist[chunk][bucket] -> prefix over earlier chunks of the same bucket;
 // bstart[b] = first position of bucket b inside the segment; bsize[b] = size.
-[[maybe_unused]] static __global__ __launch_bounds__(BMAX) void k_colscan(const PartSeg* ps, uint32_t* hist, uint64_t* bstart, uint64_t* bsize) {
+[[maybe_unused]] static __global__ __launch_bounds__(BMAX) void k_colscan(const PartSeg* ps, const uint32_t* nseg,
+                                                                         uint32_t* hist, uint64_t* bstart,
+                                                                         uint64_t* bsize) {
     __shared__ uint32_t wtot[BMAX / 64];
-    const PartSeg s = ps[blockIdx.x];
+    for (uint32_t w = blockIdx.x; w < *nseg; w += gridDim.x) {
+    const PartSeg s = ps[w];
     const uint32_t B = 2u << s.logf;
     const uint32_t t = threadIdx.x;
     uint32_t run = 0;
@@ -458,6 +478,7 @@ __global__ __launch_bounds__(CNT_P) void k_count(const PartSeg* ps, const ChunkR
         bstart[s.bkt_base + t] = start;
         bsize[s.bkt_base + t] = run;
     }
+    }
 }
 
 // ---------------------------------------------------------------- scatter
@@ -474,7 +495,8 @@ template <class T, bool KV>
 __global__ __launch_bounds__(SCAT_P) void k_scatter(const PartSeg* ps, const ChunkRef* cr, T* a, T* b,
                                                     int32_t* va, int32_t* vb,
                                                     const typename KeyTraits<T>::K* trees, const uint32_t* guides,
-                                                    const uint32_t* hist, const uint64_t* bstart, LearnedTable lt) {
+                                                    const uint32_t* hist, const uint64_t* bstart, LearnedTable lt,
+                                                    const uint32_t* nchunk) {
     using Tr = KeyTraits<T>;
     using K = typename Tr::K;
     constexpr int NW = SCAT_P / 64;
@@ -490,7 +512,8 @@ __global__ __launch_bounds__(SCAT_P) void k_scatter(const PartSeg* ps, const Chu
     __shared__ uint16_t stb[SCAT_ROUND];
     __shared__ int32_t stv[KV ? SCAT_ROUND : 1];
 
-    const ChunkRef c = cr[blockIdx.x];
+    for (uint32_t wi = blockIdx.x; wi < *nchunk; wi += gridDim.x) {
+    const ChunkRef c = cr[wi];
     const PartSeg s = ps[c.seg];
     const uint32_t F = 1u << s.logf, B = 2 * F;
     const uint32_t t = threadIdx.x, lane = t & 63u, wid = t >> 6;
@@ -590,6 +613,8 @@ __global__ __launch_bounds__(SCAT_P) void k_scatter(const PartSeg* ps, const Chu
         // staging area is rewritten only after its barrier 3, which every
         // thread reaches after finishing the stores above
     }
+    __syncthreads();  // LDS is reloaded for the next chunk
+    }
 }
 
 // ---------------------------------------------------------------- plan
@@ -607,12 +632,14 @@ struct PlanLists {
 
 // Buckets are counted per workgroup in LDS first, so the global list counters
 // see one atomic per (workgroup, list) instead of one per bucket.
-[[maybe_unused]] static __global__ __launch_bounds__(256) void k_plan(const PartSeg* ps, const uint64_t* bstart, const uint64_t* bsize,
-                                              PlanLists pl) {
+[[maybe_unused]] static __global__ __launch_bounds__(256) void k_plan(const PartSeg* ps, const uint32_t* nseg,
+                                                                     const uint64_t* bstart, const uint64_t* bsize,
+                                                                     PlanLists pl) {
     constexpr int NL = 10;  // lists: tile classes 0..7, copy = 8, next = 9
     __shared__ uint32_t lcnt[NL], lbase[NL];
     __shared__ unsigned long long lkeys[NL];
-    const PartSeg s = ps[blockIdx.x];
+    for (uint32_t w = blockIdx.x; w < *nseg; w += gridDim.x) {
+    const PartSeg s = ps[w];
     const uint32_t B = 2u << s.logf;
     const uint32_t b = blockIdx.y * blockDim.x + threadIdx.x;
     if (threadIdx.x < NL) {
@@ -652,20 +679,23 @@ struct PlanLists {
         if (l <= 8) atomicAdd((unsigned long long*)&pl.tile_keys[l], lkeys[l]);
     }
     __syncthreads();
-    if (list < 0) return;
-    const uint32_t o = lbase[list] + rank;
-    if (list < 8) {
-        pl.tile[list][o] = TileSeg{begin, (uint32_t)sz, buf};
-    } else if (list == 8) {
-        for (uint32_t p = 0; p < np; ++p) {
-            const uint64_t pb = (uint64_t)p * COPY_PIECE;
-            pl.copy[o + p] = TileSeg{begin + pb, (uint32_t)((sz - pb) < COPY_PIECE ? (sz - pb) : COPY_PIECE), 1u};
+    if (list >= 0) {
+        const uint32_t o = lbase[list] + rank;
+        if (list < 8) {
+            pl.tile[list][o] = TileSeg{begin, (uint32_t)sz, buf};
+        } else if (list == 8) {
+            for (uint32_t p = 0; p < np; ++p) {
+                const uint64_t pb = (uint64_t)p * COPY_PIECE;
+                pl.copy[o + p] = TileSeg{begin + pb, (uint32_t)((sz - pb) < COPY_PIECE ? (sz - pb) : COPY_PIECE), 1u};
+            }
+        } else {
+            // a bucket of a learned level with a finer level below keeps the
+            // learned classifier: aux = (bucket index << 1) | 1
+            const uint32_t aux = (s.mode && s.lshift) ? ((((b >> 1) + s.lbase) << 1) | 1u) : 0u;
+            pl.next[o] = Seg{begin, sz, buf, aux};
         }
-    } else {
-        // a bucket of a learned level with a finer level below keeps the
-        // learned classifier: aux = (bucket index << 1) | 1
-        const uint32_t aux = (s.mode && s.lshift) ? ((((b >> 1) + s.lbase) << 1) | 1u) : 0u;
-        pl.next[o] = Seg{begin, sz, buf, aux};
+    }
+    __syncthreads();
     }
 }
 

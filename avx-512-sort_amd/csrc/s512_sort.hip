@@ -96,90 +96,27 @@ static inline int ceil_log2(uint64_t x) {
 }
 
 // ---------------------------------------------------------------- engine
-template <class T, bool KV>
-s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const EngineParams& prm, s512_stats* stats,
-                        bool profile) {
-    if (stats) memset(stats, 0, sizeof(*stats));
-    if (n <= 1) return S512_OK;
-    if (n >= (1ull << 32)) {
-        set_error("n = %zu exceeds the supported maximum of 2^32-1 keys", n);
-        return S512_ERR_TOO_LARGE;
-    }
-    int dev;
-    s512_status rc = current_device(&dev);
-    if (rc) return rc;
-    DeviceState& ds = device_state(dev);
-    std::lock_guard<std::mutex> lk(ds.mu);
-    return sort_engine_locked<T, KV>(a, va, n, st, prm, stats, profile, ds);
-}
-
-template <class T, bool KV>
-s512_status sort_engine_locked(T* a, int32_t* va, size_t n, hipStream_t st, const EngineParams& prm,
-                               s512_stats* stats, bool profile, DeviceState& ds) {
-    using K = typename KeyTraits<T>::K;
-    using TC = TileCfg<T, KV>;
-    if (n <= 1) return S512_OK;
-    s512_status rc;
-    Profiler pf;
-    pf.on = profile && stats;
-    pf.st = st;
-    pf.ds = &ds;
-
-    const int ncls = TC::nclass();
-    // ---- one tile: no partition level at all
-    if (n <= TC::maxcap()) {
-        SizeCounter sc;
-        sc.take<TileSeg>(1);
-        sc.take<uint32_t>(1);
-        if ((rc = ds.arena.reserve(sc.off, st))) return rc;
-        if ((rc = ds.pinned.reserve(sc.off, st))) return rc;
-        Arena ar{(char*)ds.arena.ptr, 0}, hp{(char*)ds.pinned.ptr, 0};
-        TileSeg* dseg = ar.take<TileSeg>(1);
-        uint32_t* dcnt = ar.take<uint32_t>(1);
-        TileSeg* hseg = hp.take<TileSeg>(1);
-        uint32_t* hcnt = hp.take<uint32_t>(1);
-        *hseg = TileSeg{0, (uint32_t)n, 0};
-        *hcnt = 1;
-        S512_HIP(hipMemcpyAsync(ds.arena.ptr, ds.pinned.ptr, sc.off, hipMemcpyHostToDevice, st));
-        pf.begin(S512_PHASE_TILE);
-        launch_tile<T, KV>(TC::cls_of(n), dseg, dcnt, 1, a, va, a, va, st);
-        S512_CHECK_LAUNCH();
-        pf.end();
-        S512_HIP(hipStreamSynchronize(st));  // pinned staging is reused by the next call
-        if (stats) {
-            stats->tiles = 1;
-            stats->keys[S512_PHASE_TILE] = n;
-            pf.collect(stats->kernel_ms, stats->launches);
-        }
-        return S512_OK;
-    }
-
-    // ---- learned partitioner: model of the key CDF for the first two levels
-    LearnedModel model;
-    memset(&model, 0, sizeof(model));
+// State of the learned partitioner across the level loop.
+struct LevelState {
     bool use_learned = false;
     uint32_t lf1 = 0, lf2 = 0;
-    if constexpr (!KV && (std::is_same<T, double>::value || std::is_same<T, int32_t>::value)) {
-        if (prm.learned) {
-            if ((rc = build_learned_model<T>(a, n, st, ds, prm, &model, stats))) return rc;
-            use_learned = !model.degenerate;
-            const uint64_t leaves = (n + prm.target_leaf - 1) / prm.target_leaf;
-            const int total = std::max(1, ceil_log2(leaves));
-            lf1 = (uint32_t)std::min(total, prm.max_logf);
-            lf2 = (uint32_t)std::max(0, std::min(total - (int)lf1, prm.max_logf));
-            model.table.nb_log = lf1 + lf2;
-        }
-    }
+};
+
+// The partition level loop over an initial list of segments of the arrays
+// a (buf 0) and b (buf 1), both n keys long (key/value: va, vb).  Synchronises
+// the stream between levels; returns with every segment sorted into a.
+template <class T, bool KV>
+s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vector<Seg> segs, hipStream_t st,
+                       const EngineParams& prm, s512_stats* stats, Profiler& pf, DeviceState& ds,
+                       const LearnedTable& ltab, LevelState& lstate) {
+    using K = typename KeyTraits<T>::K;
+    using TC = TileCfg<T, KV>;
+    s512_status rc;
+    const int ncls = TC::nclass();
+    bool& use_learned = lstate.use_learned;
+    const uint32_t lf1 = lstate.lf1, lf2 = lstate.lf2;
     bool learned_checked = false;
-
-    // ---- scratch arrays (same size as the input)
-    const size_t dbytes = n * sizeof(T) + (KV ? n * sizeof(int32_t) : 0);
-    if ((rc = ds.data.reserve(dbytes, st))) return rc;
-    T* b = (T*)ds.data.ptr;
-    int32_t* vb = KV ? (int32_t*)((char*)ds.data.ptr + n * sizeof(T)) : nullptr;
-
-    // aux = 1: the segment is partitioned by the learned model (first level)
-    std::vector<Seg> segs{Seg{0, n, 0, use_learned ? 1u : 0u}};
+    struct { LearnedTable table; } model{ltab};
     std::vector<PartSeg> ps;
     std::vector<ChunkRef> crs;
     const uint32_t smax = 8192;
@@ -237,6 +174,8 @@ s512_status sort_engine_locked(T* a, int32_t* va, size_t n, hipStream_t st, cons
             p.kmin = 0;
             p.shift = 0;
             if (p.mode) p.nsample = 0;
+            p.pad = 0;
+            p.sample_base = (uint64_t)i * smax;
             sumF += F;
             sumG += (uint64_t)GUIDE_PER_SPLIT * F;
             histN += (uint64_t)p.nchunks * B;
@@ -306,6 +245,8 @@ s512_status sort_engine_locked(T* a, int32_t* va, size_t n, hipStream_t st, cons
         memset(h_cnt, 0, 32 * sizeof(uint32_t));
         memset(h_cnt64, 0, 16 * sizeof(uint64_t));
         h_cnt[10] = nsplit;
+        h_cnt[12] = m;   // segments of this level
+        h_cnt[13] = tc;  // chunks of this level
         S512_HIP(hipMemcpyAsync(ds.arena.ptr, ds.pinned.ptr, up_bytes, hipMemcpyHostToDevice, st));
 
         const uint64_t seed = prm.seed + 0x9E37ull * (uint64_t)level;
@@ -313,7 +254,7 @@ s512_status sort_engine_locked(T* a, int32_t* va, size_t n, hipStream_t st, cons
         for (auto& p : ps) (p.mode ? any_learned : any_split) = true;
         if (any_split) {
         pf.begin(S512_PHASE_SAMPLE);
-        k_sample<T><<<m, 256, 0, st>>>(d_ps, a, b, d_smp, smax, seed);
+        k_sample<T><<<m, 256, 0, st>>>(d_ps, d_cnt + 12, a, b, d_smp, smax, seed);
         S512_CHECK_LAUNCH();
         pf.end();
         pf.begin(S512_PHASE_SAMPLE_SORT);
@@ -321,16 +262,17 @@ s512_status sort_engine_locked(T* a, int32_t* va, size_t n, hipStream_t st, cons
         S512_CHECK_LAUNCH();
         pf.end();
         pf.begin(S512_PHASE_TREE);
-        k_build_tree<K><<<m, FMAX, 0, st>>>(d_ps, d_smp, smax, d_tree, d_guide);
+        k_build_tree<K><<<m, FMAX, 0, st>>>(d_ps, d_cnt + 12, d_smp, smax, d_tree, d_guide);
         S512_CHECK_LAUNCH();
         pf.end();
         }
         pf.begin(S512_PHASE_COUNT);
-        k_count<T><<<tc, CNT_P, 0, st>>>(d_ps, d_cr, a, b, d_tree, d_guide, d_hist, d_cnt + 11, model.table);
+        k_count<T><<<tc, CNT_P, 0, st>>>(d_ps, d_cr, d_cnt + 13, a, b, d_tree, d_guide, d_hist, d_cnt + 11,
+                                         model.table);
         S512_CHECK_LAUNCH();
         pf.end();
         pf.begin(S512_PHASE_SCAN);
-        k_colscan<<<m, BMAX, 0, st>>>(d_ps, d_hist, d_bstart, d_bsize);
+        k_colscan<<<m, BMAX, 0, st>>>(d_ps, d_cnt + 12, d_hist, d_bstart, d_bsize);
         S512_CHECK_LAUNCH();
         pf.end();
         if (any_learned && level == 1 && !learned_checked) {
@@ -357,11 +299,11 @@ s512_status sort_engine_locked(T* a, int32_t* va, size_t n, hipStream_t st, cons
         }
         pf.begin(S512_PHASE_SCATTER);
         k_scatter<T, KV><<<tc, SCAT_P, 0, st>>>(d_ps, d_cr, a, b, va, vb, d_tree, d_guide, d_hist, d_bstart,
-                                                model.table);
+                                                model.table, d_cnt + 13);
         S512_CHECK_LAUNCH();
         pf.end();
         pf.begin(S512_PHASE_PLAN);
-        k_plan<<<dim3(m, (uint32_t)((maxB + 255) / 256)), 256, 0, st>>>(d_ps, d_bstart, d_bsize, pl);
+        k_plan<<<dim3(m, (uint32_t)((maxB + 255) / 256)), 256, 0, st>>>(d_ps, d_cnt + 12, d_bstart, d_bsize, pl);
         S512_CHECK_LAUNCH();
         pf.end();
         keys_ph[S512_PHASE_COUNT] += total;
@@ -414,6 +356,92 @@ s512_status sort_engine_locked(T* a, int32_t* va, size_t n, hipStream_t st, cons
         pf.collect(stats->kernel_ms, stats->launches);
     }
     return S512_OK;
+}
+
+template <class T, bool KV>
+s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const EngineParams& prm, s512_stats* stats,
+                        bool profile) {
+    if (stats) memset(stats, 0, sizeof(*stats));
+    if (n <= 1) return S512_OK;
+    if (n >= (1ull << 32)) {
+        set_error("n = %zu exceeds the supported maximum of 2^32-1 keys", n);
+        return S512_ERR_TOO_LARGE;
+    }
+    int dev;
+    s512_status rc = current_device(&dev);
+    if (rc) return rc;
+    DeviceState& ds = device_state(dev);
+    std::lock_guard<std::mutex> lk(ds.mu);
+    return sort_engine_locked<T, KV>(a, va, n, st, prm, stats, profile, ds);
+}
+
+template <class T, bool KV>
+s512_status sort_engine_locked(T* a, int32_t* va, size_t n, hipStream_t st, const EngineParams& prm,
+                               s512_stats* stats, bool profile, DeviceState& ds) {
+    using TC = TileCfg<T, KV>;
+    if (n <= 1) return S512_OK;
+    s512_status rc;
+    Profiler pf;
+    pf.on = profile && stats;
+    pf.st = st;
+    pf.ds = &ds;
+
+    // ---- one tile: no partition level at all
+    if (n <= TC::maxcap()) {
+        SizeCounter sc;
+        sc.take<TileSeg>(1);
+        sc.take<uint32_t>(1);
+        if ((rc = ds.arena.reserve(sc.off, st))) return rc;
+        if ((rc = ds.pinned.reserve(sc.off, st))) return rc;
+        Arena ar{(char*)ds.arena.ptr, 0}, hp{(char*)ds.pinned.ptr, 0};
+        TileSeg* dseg = ar.take<TileSeg>(1);
+        uint32_t* dcnt = ar.take<uint32_t>(1);
+        TileSeg* hseg = hp.take<TileSeg>(1);
+        uint32_t* hcnt = hp.take<uint32_t>(1);
+        *hseg = TileSeg{0, (uint32_t)n, 0};
+        *hcnt = 1;
+        S512_HIP(hipMemcpyAsync(ds.arena.ptr, ds.pinned.ptr, sc.off, hipMemcpyHostToDevice, st));
+        pf.begin(S512_PHASE_TILE);
+        launch_tile<T, KV>(TC::cls_of(n), dseg, dcnt, 1, a, va, a, va, st);
+        S512_CHECK_LAUNCH();
+        pf.end();
+        S512_HIP(hipStreamSynchronize(st));  // pinned staging is reused by the next call
+        if (stats) {
+            stats->tiles = 1;
+            stats->keys[S512_PHASE_TILE] = n;
+            pf.collect(stats->kernel_ms, stats->launches);
+        }
+        return S512_OK;
+    }
+
+    // ---- learned partitioner: model of the key CDF for the first two levels
+    LearnedModel model;
+    memset(&model, 0, sizeof(model));
+    bool use_learned = false;
+    uint32_t lf1 = 0, lf2 = 0;
+    if constexpr (!KV && (std::is_same<T, double>::value || std::is_same<T, int32_t>::value)) {
+        if (prm.learned) {
+            if ((rc = build_learned_model<T>(a, n, st, ds, prm, &model, stats))) return rc;
+            use_learned = !model.degenerate;
+            const uint64_t leaves = (n + prm.target_leaf - 1) / prm.target_leaf;
+            const int total = std::max(1, ceil_log2(leaves));
+            lf1 = (uint32_t)std::min(total, prm.max_logf);
+            lf2 = (uint32_t)std::max(0, std::min(total - (int)lf1, prm.max_logf));
+            model.table.nb_log = lf1 + lf2;
+        }
+    }
+    // ---- scratch arrays (same size as the input)
+    const size_t dbytes = n * sizeof(T) + (KV ? n * sizeof(int32_t) : 0);
+    if ((rc = ds.data.reserve(dbytes, st))) return rc;
+    T* b = (T*)ds.data.ptr;
+    int32_t* vb = KV ? (int32_t*)((char*)ds.data.ptr + n * sizeof(T)) : nullptr;
+
+    std::vector<Seg> segs{Seg{0, n, 0, use_learned ? 1u : 0u}};
+    LevelState ls;
+    ls.use_learned = use_learned;
+    ls.lf1 = lf1;
+    ls.lf2 = lf2;
+    return run_levels<T, KV>(a, va, b, vb, n, segs, st, prm, stats, pf, ds, model.table, ls);
 }
 
 template s512_status sort_engine<int32_t, false>(int32_t*, int32_t*, size_t, hipStream_t, const EngineParams&, s512_stats*, bool);
