@@ -1,0 +1,36 @@
+// s512_engine_internal.hpp -- pieces of the sort engine shared with the
+// segmented sort: the partition level loop and the plain-key tile launches.
+#pragma once
+#include <vector>
+#include "s512_kernels.hpp"
+#include "s512_learned.hpp"
+#include "s512_runtime.hpp"
+#include "s512_engine.hpp"
+
+namespace s512 {
+
+// State of the learned partitioner across the level loop.
+struct LevelState {
+    bool use_learned = false;
+    uint32_t lf1 = 0, lf2 = 0;
+};
+
+// The partition level loop over an initial list of segments of the arrays
+// a (buf 0) and b (buf 1), both n keys long (key/value: va, vb).  Synchronises
+// the stream between levels; returns with every segment sorted into a.
+template <class T, bool KV>
+s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vector<Seg> segs, hipStream_t st,
+                       const EngineParams& prm, s512_stats* stats, Profiler& pf, DeviceState& ds,
+                       const LearnedTable& ltab, LevelState& lstate);
+
+// Capacities (keys) of the tile classes for plain keys of type T.
+template <class T>
+const uint32_t* tile_caps(int* ncls);
+
+// Launch the tile sort of class cls over the device-counted list segs with a
+// grid sized for grid_items segments (the kernels loop over the true count).
+template <class T>
+void launch_tile_fixed(int cls, const TileSeg* segs, const uint32_t* dcount, uint32_t grid_items, T* a, const T* b,
+                       hipStream_t st);
+
+}  // namespace s512

@@ -628,6 +628,8 @@ struct PlanLists {
     uint32_t* copy_count;
     Seg* next;                // segments for the next partition level
     uint32_t* next_count;
+    uint32_t cap[10];         // entries each list holds (tile classes, copy, next)
+    uint32_t* overflow;       // set when an append would pass its list's capacity
 };
 
 // Buckets are counted per workgroup in LDS first, so the global list counters
@@ -677,9 +679,10 @@ struct PlanLists {
         uint32_t* cnt = l < 8 ? &pl.tile_count[l] : (l == 8 ? pl.copy_count : pl.next_count);
         lbase[l] = atomicAdd(cnt, lcnt[l]);
         if (l <= 8) atomicAdd((unsigned long long*)&pl.tile_keys[l], lkeys[l]);
+        if ((uint64_t)lbase[l] + lcnt[l] > pl.cap[l]) atomicOr(pl.overflow, 1u);
     }
     __syncthreads();
-    if (list >= 0) {
+    if (list >= 0 && (uint64_t)lbase[list] + rank + np <= pl.cap[list]) {
         const uint32_t o = lbase[list] + rank;
         if (list < 8) {
             pl.tile[list][o] = TileSeg{begin, (uint32_t)sz, buf};

@@ -15,6 +15,7 @@
 #include "s512_engine.hpp"
 #include "s512_tile_f64.hpp"
 #include "s512_learned_host.hpp"
+#include "s512_engine_internal.hpp"
 #include <algorithm>
 #include <type_traits>
 
@@ -95,13 +96,25 @@ static inline int ceil_log2(uint64_t x) {
     return l;
 }
 
-// ---------------------------------------------------------------- engine
-// State of the learned partitioner across the level loop.
-struct LevelState {
-    bool use_learned = false;
-    uint32_t lf1 = 0, lf2 = 0;
-};
+// Tile classes of plain keys, for callers outside this file.
+template <class T>
+const uint32_t* tile_caps(int* ncls) {
+    *ncls = TileCfg<T, false>::nclass();
+    return TileCfg<T, false>::caps();
+}
+template <class T>
+void launch_tile_fixed(int cls, const TileSeg* segs, const uint32_t* dcount, uint32_t grid_items, T* a, const T* b,
+                       hipStream_t st) {
+    launch_tile_keys<T>(cls, segs, dcount, grid_items, a, b, st);
+}
+template const uint32_t* tile_caps<int32_t>(int*);
+template const uint32_t* tile_caps<double>(int*);
+template void launch_tile_fixed<int32_t>(int, const TileSeg*, const uint32_t*, uint32_t, int32_t*, const int32_t*,
+                                         hipStream_t);
+template void launch_tile_fixed<double>(int, const TileSeg*, const uint32_t*, uint32_t, double*, const double*,
+                                        hipStream_t);
 
+// ---------------------------------------------------------------- engine
 // The partition level loop over an initial list of segments of the arrays
 // a (buf 0) and b (buf 1), both n keys long (key/value: va, vb).  Synchronises
 // the stream between levels; returns with every segment sorted into a.
@@ -231,6 +244,10 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
         pl.copy_count = d_cnt + 8;
         pl.next = ar.take<Seg>(sumB);
         pl.next_count = d_cnt + 9;
+        for (int c = 0; c < ncls; ++c) pl.cap[c] = (uint32_t)sumB;
+        pl.cap[8] = (uint32_t)copy_cap;
+        pl.cap[9] = (uint32_t)sumB;
+        pl.overflow = d_cnt + 14;
 
         PartSeg* h_ps = hp.take<PartSeg>(m);
         ChunkRef* h_cr = hp.take<ChunkRef>(tc);
@@ -315,6 +332,10 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
         uint64_t counts64[16];
         memcpy(counts, h_out, sizeof(counts));
         memcpy(counts64, (char*)h_out + ((char*)d_cnt64 - (char*)d_cnt), sizeof(counts64));
+        if (counts[14]) {
+            set_error("plan lists overflowed at level %d", level);
+            return S512_ERR_INTERNAL;
+        }
         // level 1 saw every key: fp64 leaves may use min/max when no key is NaN or -0.0
         if (level == 1) f64_fast = !KV && std::is_same<T, double>::value && prm.f64_minmax && counts[11] == 0;
         if (counts[8]) {
@@ -444,6 +465,12 @@ s512_status sort_engine_locked(T* a, int32_t* va, size_t n, hipStream_t st, cons
     return run_levels<T, KV>(a, va, b, vb, n, segs, st, prm, stats, pf, ds, model.table, ls);
 }
 
+template s512_status run_levels<int32_t, false>(int32_t*, int32_t*, int32_t*, int32_t*, size_t, std::vector<Seg>,
+                                                hipStream_t, const EngineParams&, s512_stats*, Profiler&, DeviceState&,
+                                                const LearnedTable&, LevelState&);
+template s512_status run_levels<double, false>(double*, int32_t*, double*, int32_t*, size_t, std::vector<Seg>,
+                                               hipStream_t, const EngineParams&, s512_stats*, Profiler&, DeviceState&,
+                                               const LearnedTable&, LevelState&);
 template s512_status sort_engine<int32_t, false>(int32_t*, int32_t*, size_t, hipStream_t, const EngineParams&, s512_stats*, bool);
 template s512_status sort_engine_locked<uint64_t, false>(uint64_t*, int32_t*, size_t, hipStream_t, const EngineParams&, s512_stats*, bool, DeviceState&);
 template s512_status sort_engine_locked<uint32_t, false>(uint32_t*, int32_t*, size_t, hipStream_t, const EngineParams&, s512_stats*, bool, DeviceState&);

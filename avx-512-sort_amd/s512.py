@@ -74,6 +74,13 @@ _SIGS = {
     "s512_merge_inplace_f64_dev": ([_vp, _c_size, _c_size, _vp], ctypes.c_int),
     "s512_mlp_nparams": ([], ctypes.c_uint32),
     "s512_mlp_loss_grad_f32": ([_vp, _vp, _vp, ctypes.c_uint32, _vp, _vp, _vp], ctypes.c_int),
+    "s512_segsort_i32": ([_vp, _vp, _c_size], ctypes.c_int),
+    "s512_segsort_f64": ([_vp, _vp, _c_size], ctypes.c_int),
+    "s512_segsort_i32_dev": ([_vp, _vp, _c_size, _vp, _vp], ctypes.c_int),
+    "s512_segsort_f64_dev": ([_vp, _vp, _c_size, _vp, _vp], ctypes.c_int),
+    "s512_segsort_plan_create": ([ctypes.c_int, _vp, _vp, _c_size, _c_size, ctypes.POINTER(_vp)], ctypes.c_int),
+    "s512_segsort_plan_launch": ([_vp, _vp, _vp], ctypes.c_int),
+    "s512_segsort_plan_destroy": ([_vp], ctypes.c_int),
     "s512x_fill_f64": ([_vp, _c_size, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, _vp], ctypes.c_int),
     "s512x_fill_i32": ([_vp, _c_size, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, _vp], ctypes.c_int),
 }
@@ -303,3 +310,72 @@ def merge_inplace(a, center):
     else:
         _check(getattr(lib, f"s512_merge_inplace_{k}")(_ptr(a), _n(a), center))
     return a
+
+
+def _offsets_ok(keys, offsets):
+    if _is_torch(offsets):
+        import torch
+        if offsets.dtype not in (torch.int64, torch.uint64):
+            raise TypeError("offsets must be int64 / uint64")
+        if _dev(offsets) != _dev(keys):
+            raise ValueError("keys and offsets must both be on the device or both on the host")
+    else:
+        if np.dtype(offsets.dtype) not in (np.dtype(np.int64), np.dtype(np.uint64)):
+            raise TypeError("offsets must be int64 / uint64")
+    if _n(offsets) < 1:
+        raise ValueError("offsets holds nseg + 1 entries")
+
+
+def segsort(keys, offsets, stats=False):
+    """Sort every segment keys[offsets[i]:offsets[i+1]] independently, in place."""
+    k = _kind(keys)
+    _offsets_ok(keys, offsets)
+    nseg = _n(offsets) - 1
+    if _dev(keys):
+        st = Stats()
+        _check(getattr(lib, f"s512_segsort_{k}_dev")(_ptr(keys), _ptr(offsets), nseg, ctypes.byref(st),
+                                                     _stream_of(keys)))
+        return st if stats else keys
+    _check(getattr(lib, f"s512_segsort_{k}")(_ptr(keys), _ptr(offsets), nseg))
+    return keys
+
+
+class SegsortPlan:
+    """A captured segmented sort for fixed device tensors (keys, offsets)."""
+
+    def __init__(self, keys, offsets):
+        k = _kind(keys)
+        _offsets_ok(keys, offsets)
+        if not _dev(keys):
+            raise ValueError("a plan needs device tensors")
+        self.keys, self.offsets = keys, offsets  # keep them alive
+        self.h = _vp()
+        _check(lib.s512_segsort_plan_create(0 if k == "i32" else 1, _ptr(keys), _ptr(offsets), _n(offsets) - 1,
+                                            _n(keys), ctypes.byref(self.h)))
+
+    def launch(self):
+        st = Stats()
+        _check(lib.s512_segsort_plan_launch(self.h, ctypes.byref(st), _stream_of(self.keys)))
+        return st
+
+    def close(self):
+        if self.h:
+            lib.s512_segsort_plan_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def lognormal_offsets(nseg, median=512.0, sigma=1.5, lo=1, hi=1 << 20, seed=0):
+    """Offsets (uint64, nseg + 1 entries) of nseg segments whose sizes are
+    lognormal(median, sigma) rounded and clipped to [lo, hi]; numpy's PCG64
+    seeded with `seed` (synthetic batched-sort workload)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.clip(np.rint(median * np.exp(sigma * rng.standard_normal(nseg))), lo, hi).astype(np.uint64)
+    off = np.zeros(nseg + 1, dtype=np.uint64)
+    np.cumsum(sizes, out=off[1:])
+    return off

@@ -177,6 +177,40 @@ s512_status s512kv_partition_i32_dev(int32_t* d_keys, int32_t* d_values, size_t 
 /* Sort512kv::SmallSort16V(int*, int*, len <= 256)   sort512kv.hpp:5235     */
 s512_status s512kv_smallsort16v_i32(int32_t* keys, int32_t* values, size_t len);
 
+/* --------------------------------------------------------- segmented sort */
+/* Many independent arrays in one call.  No single reference entry point: it
+ * replaces a caller's loop of SmallSort16V (sort512perf.cpp:656-661 sorts many
+ * small arrays that way) or of Sort512::Sort (sort512.hpp:5960) per array.
+ * Segment i is keys[offsets[i] .. offsets[i+1]) (offsets has nseg + 1
+ * entries, non-decreasing); each segment is sorted in place, independently,
+ * with the same key order as s512_sort_*.  Keys outside every segment are
+ * left alone.  Offsets that decrease return S512_ERR_INVALID with the keys
+ * unchanged.  At most 2^31-1 segments and 2^32-1 keys per call.
+ * One fixed kernel sequence sorts every segment up to the largest tile (16384
+ * int32 / 8192 double keys) and partitions larger ones once; only buckets
+ * still above a tile then need the host-driven engine.                     */
+enum { S512_DTYPE_I32 = 0, S512_DTYPE_F64 = 1 };
+s512_status s512_segsort_i32(int32_t* keys, const uint64_t* offsets, size_t nseg);
+s512_status s512_segsort_f64(double* keys, const uint64_t* offsets, size_t nseg);
+/* Device pointers on `stream`; stats (may be NULL): levels = 1 when a segment
+ * was partitioned, tiles / keys[S512_PHASE_TILE] = tiles sorted and their keys,
+ * learned_levels = buckets finished by the multi-level engine.             */
+s512_status s512_segsort_i32_dev(int32_t* d_keys, const uint64_t* d_offsets, size_t nseg, s512_stats* stats,
+                                 void* stream);
+s512_status s512_segsort_f64_dev(double* d_keys, const uint64_t* d_offsets, size_t nseg, s512_stats* stats,
+                                 void* stream);
+/* Reusable plan: the kernel sequence for fixed (d_keys, d_offsets, nseg,
+ * nkeys = length of d_keys) captured once in a hipGraph, with its own
+ * workspace.  Launch replays the graph on `stream` (the offsets and keys may
+ * change between launches, the pointers and sizes may not), then reads one
+ * control block back and finishes oversized buckets, if any.  A plan is used
+ * by one thread at a time.                                                 */
+typedef struct s512_segsort_plan_t* s512_segsort_plan;
+s512_status s512_segsort_plan_create(int dtype, void* d_keys, const uint64_t* d_offsets, size_t nseg, size_t nkeys,
+                                     s512_segsort_plan* out);
+s512_status s512_segsort_plan_launch(s512_segsort_plan plan, s512_stats* stats, void* stream);
+s512_status s512_segsort_plan_destroy(s512_segsort_plan plan);
+
 #ifdef __cplusplus
 }
 #endif

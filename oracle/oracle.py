@@ -37,6 +37,8 @@ def _load():
         "orc_sort_i32": ([_vp, _sz], ctypes.c_int),
         "orc_sort_f64": ([_vp, _sz], ctypes.c_int),
         "orc_sort_kv": ([_vp, _vp, _sz], ctypes.c_int),
+        "orc_segsort_i32": ([_vp, _vp, _sz], ctypes.c_int),
+        "orc_segsort_f64": ([_vp, _vp, _sz], ctypes.c_int),
         "orc_qsort_i32": ([_vp, _sz], None),
         "orc_qsort_f64": ([_vp, _sz], None),
         "orc_pair_multiset_kv": ([_vp, _vp, _sz, _vp], None),
@@ -144,6 +146,14 @@ def kv_smallsort(k, v):
     rc = lib.orc_smallsort_kv(_p(k), _p(v), k.size)
     assert rc == 0
     return k, v
+
+
+def segsort(a, off):
+    """Loop of Sort512::Sort over the segments a[off[i]:off[i+1]], in place."""
+    off = np.ascontiguousarray(off, dtype=np.uint64)
+    rc = getattr(lib, "orc_segsort_" + _suffix(a))(_p(a), _p(off), off.size - 1)
+    assert rc == 0
+    return a
 
 
 def closed_sort(a):

@@ -223,6 +223,20 @@ static void small_f64(double *a, size_t len) { orc_smallsort_f64(a, len); }
 DEFINE_QSORT(orc_sort_i32, int32_t, 256, small_i32, orc_partition_i32)
 DEFINE_QSORT(orc_sort_f64, double, 128, small_f64, orc_partition_f64)
 
+/* Segmented sort: the caller's loop of Sort512::Sort (sort512.hpp:5960) over
+ * the segments a[off[i] .. off[i+1]); returns -1 on decreasing offsets. */
+#define DEFINE_SEGSORT(NAME, T, SORT)                                         \
+    int NAME(T *a, const uint64_t *off, size_t nseg)                          \
+    {                                                                         \
+        for (size_t i = 0; i < nseg; ++i)                                     \
+            if (off[i + 1] < off[i]) return -1;                               \
+        for (size_t i = 0; i < nseg; ++i)                                     \
+            if (SORT(a + off[i], (size_t)(off[i + 1] - off[i]))) return -1;   \
+        return 0;                                                             \
+    }
+DEFINE_SEGSORT(orc_segsort_i32, int32_t, orc_sort_i32)
+DEFINE_SEGSORT(orc_segsort_f64, double, orc_sort_f64)
+
 static size_t kv_pivot(const int32_t *a, size_t l, size_t r)
 {
     size_t m = ((r - l) / 2) + l;
