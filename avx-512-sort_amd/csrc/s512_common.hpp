@@ -75,7 +75,7 @@ struct PartSeg {
     uint32_t mode;       // 0: sampled splitters, 1: learned model
     uint32_t lshift;     // learned: bucket = (fine bucket >> lshift) - lbase
     uint32_t lbase;
-    uint32_t pad;
+    uint32_t flags;      // classifier flags (GUIDE_SUBTABLES, GUIDE_EQBUCKETS), set on the device
     uint64_t sample_base;  // offset of this segment's samples in the sample buffer
 };
 

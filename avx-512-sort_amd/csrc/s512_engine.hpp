@@ -20,6 +20,10 @@ struct EngineParams {
     float learn_rate;        // learned: Adam step size
     float imbalance;         // learned: fall back when a level-1 bucket exceeds this x mean
     uint64_t seed;           // sample positions
+    int scatter_shape;       // k_scatter workgroup shape (0 .. SCATTER_SHAPES-1)
+    bool ids;                // count stores each key's bucket; scatter reads it
+    int count_shape;         // k_count shape (0 .. COUNT_SHAPES-1)
+    int tile_plan;           // tile sort shapes (0 .. TILE_PLANS-1)
 };
 
 EngineParams default_params(bool wide_keys);

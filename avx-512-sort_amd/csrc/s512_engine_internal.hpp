@@ -30,7 +30,7 @@ const uint32_t* tile_caps(int* ncls);
 // Launch the tile sort of class cls over the device-counted list segs with a
 // grid sized for grid_items segments (the kernels loop over the true count).
 template <class T>
-void launch_tile_fixed(int cls, const TileSeg* segs, const uint32_t* dcount, uint32_t grid_items, T* a, const T* b,
-                       hipStream_t st);
+void launch_tile_fixed(int cls, int plan, const TileSeg* segs, const uint32_t* dcount, uint32_t grid_items, T* a,
+                       const T* b, hipStream_t st);
 
 }  // namespace s512

@@ -28,12 +28,11 @@ namespace s512 {
 constexpr int FMAX_LOG = 9;                 // at most 512-way fan-out per level
 constexpr int FMAX = 1 << FMAX_LOG;
 constexpr int BMAX = 2 * FMAX;              // ordinary + equality buckets
-constexpr int SCAT_P = 512;                 // scatter workgroup
-constexpr int SCAT_E = 16;                  // keys per thread per round
-constexpr int SCAT_ROUND = SCAT_P * SCAT_E; // keys staged in LDS per round
-constexpr int CNT_P = 256;                  // count workgroup
-constexpr int CNT_U = 16;                   // loads in flight per thread
+constexpr int SCAT_ROUND = 8192;            // chunks are multiples of this many keys
+constexpr int SCATTER_SHAPES = 3;           // (threads, keys per thread): 512x16, 256x16, 1024x8
+constexpr int COUNT_SHAPES = 5;            // (threads, keys per batch, classified together), see launch_count
 constexpr uint32_t COPY_PIECE = 1u << 16;   // keys per copy work item
+constexpr int CLS_NB = 8;                   // keys classified together (overlapped LDS reads)
 
 struct TileSeg {
     uint64_t begin;
@@ -75,38 +74,79 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wtot, 
 }
 
 // ---------------------------------------------------------------- classify
-// Splitters of a segment: srt[0 .. F-1) sorted and de-duplicated, padded with
-// KMAX, srt[F-1] = KMAX.  Bucket of key x: j = #{srt < x} ordinary bucket 2j,
-// or the equality bucket 2j+1 when x == srt[j].  j is found with a guide
-// table over the segment's encoded key range: cell c = (x - kmin) >> shift
-// (clamped) stores [j0, j1], the values j can take for keys in the cell, and a
-// binary search inside that range finishes (usually zero or one step).
-constexpr int GUIDE_PER_SPLIT = 8;
-constexpr int GMAX = GUIDE_PER_SPLIT * FMAX;
+// Splitters of a segment: srt[0 .. F-1) sorted and de-duplicated (nuniq real
+// ones, then KMAX pads), srt[F-1] = KMAX.  Bucket of key x: j = #{srt < x}
+// (at most nuniq), ordinary bucket 2j, or the equality bucket 2j+1 when
+// x == srt[j].
+//
+// j is found with a two-level guide over the encoded key range [kmin, kmax]
+// of the segment's sample.  The guide only narrows the search: every key,
+// inside that range or not, gets its exact j.
+//   top[c], c = (x - kmin) >> shift clamped to [0, TOP-1]:
+//     j0 | j1 << 16    j lies in [j0, j1] for every key of the cell, or
+//     SUBF | h         more than GUIDE_DENSE splitters fall in the cell:
+//   hdr[h] = {origin lo, origin hi, shift2, base | k << 16}: a sub-table of
+//     2^k cells starting at origin = the cell's first splitter; its cell
+//     q = (x - origin) >> shift2 (clamped) holds j0 | j1 << 16 as above.
+// Encoded doubles are log-like, so a normal or lognormal sample crowds most
+// splitters into a few top cells; their sub-tables spread them out again and
+// the binary search in [j0, j1] stays at zero to two steps.  A segment whose
+// top cells are all direct (uniform-like keys) skips the sub-table stage.
+//
+// Equality buckets are used only when the sample repeats a splitter value
+// (nuniq < F-1, i.e. some key is frequent enough to span a bucket).  Without
+// them a key equal to srt[j] goes to ordinary bucket 2j, which holds the keys
+// in (srt[j-1], srt[j]]: buckets stay ordered either way.
+constexpr uint32_t GUIDE_TOP_MAX = 2048;
+constexpr uint32_t GUIDE_SUBTABLES = 1u;  // PartSeg::flags: some top cell has a sub-table
+constexpr uint32_t GUIDE_EQBUCKETS = 2u;  // PartSeg::flags: keys equal to a splitter use 2j+1
+constexpr uint32_t GUIDE_DENSE = 4;
+constexpr uint32_t GUIDE_SUBF = 0x80000000u;
+__host__ __device__ constexpr uint32_t guide_top(uint32_t F) { return 8 * F < GUIDE_TOP_MAX ? 8 * F : GUIDE_TOP_MAX; }
+__host__ __device__ constexpr uint32_t guide_hdrs(uint32_t F) { return F / (GUIDE_DENSE + 1) + 1; }
+__host__ __device__ constexpr uint32_t guide_words(uint32_t F) { return guide_top(F) + 4 * guide_hdrs(F) + 8 * F; }
+constexpr uint32_t GMAX = guide_words(FMAX);
 
 struct Guide {
     uint64_t kmin;
     uint32_t shift;
-    uint32_t gmask;  // cells - 1
+    uint32_t topmask;  // top cells - 1
+    uint32_t hdr;      // word offset of the headers
+    uint32_t sub;      // word offset of the sub-tables
+    uint32_t flags;    // GUIDE_SUBTABLES | GUIDE_EQBUCKETS
 };
 
+__host__ __device__ inline Guide make_guide(uint64_t kmin, uint32_t shift, uint32_t F, uint32_t flags) {
+    const uint32_t top = guide_top(F);
+    return Guide{kmin, shift, top - 1, top, top + 4 * guide_hdrs(F), flags};
+}
+
+// Range [j0, j1] (packed) that holds j for key x.
 template <class K>
-__device__ __forceinline__ uint32_t guide_cell(const Guide& g, K x) {
-    if ((uint64_t)x < g.kmin) return 0;
-    const uint64_t c = ((uint64_t)x - g.kmin) >> g.shift;
-    return c > g.gmask ? g.gmask : (uint32_t)c;
+__device__ __forceinline__ uint32_t guide_range(const uint32_t* gd, const Guide& g, K x) {
+    const uint64_t xv = (uint64_t)x;
+    const uint64_t c = xv < g.kmin ? 0 : (xv - g.kmin) >> g.shift;
+    uint32_t e = gd[c > g.topmask ? g.topmask : (uint32_t)c];
+    if (e & GUIDE_SUBF) {
+        const uint4 h = *reinterpret_cast<const uint4*>(gd + g.hdr + 4 * (e & 0xFFFFu));
+        const uint64_t org = (uint64_t)h.x | ((uint64_t)h.y << 32);
+        const uint32_t m = (1u << (h.w >> 16)) - 1u;
+        const uint64_t q = xv < org ? 0 : (xv - org) >> h.z;
+        e = gd[g.sub + (h.w & 0xFFFFu) + (q > m ? m : (uint32_t)q)];
+    }
+    return e;
 }
 
 template <class K>
 __device__ __forceinline__ uint32_t classify(const K* srt, const uint32_t* guide, const Guide& g, K x) {
-    const uint32_t e = guide[guide_cell(g, x)];
-    uint32_t lo = e & 0xFFFFu, hi = e >> 16;
+    const uint32_t e = guide_range(guide, g, x);
+    uint32_t lo = e & 0xFFFFu, hi = (e >> 16) & 0x7FFFu;
     while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
         if (srt[mid] < x) lo = mid + 1;
         else hi = mid;
     }
-    return 2 * lo + (srt[lo] == x ? 1u : 0u);
+    return 2 * lo + ((g.flags & GUIDE_EQBUCKETS) && srt[lo] == x ? 1u : 0u);
 }
 
 template <class K>
@@ -128,12 +168,12 @@ __device__ __forceinline__ Guide load_classifier(const PartSeg& s, const K* tree
                                                  uint32_t* lguide, const LearnedTable& lt) {
     if (s.mode) {
         for (uint32_t j = threadIdx.x; j <= LEARN_G; j += NT) lguide[j] = __float_as_uint(lt.tab[j]);
-        return Guide{0, 0, 0};
+        return Guide{0, 0, 0, 0, 0, 0};
     }
-    const uint32_t F = 1u << s.logf, G = GUIDE_PER_SPLIT * F;
+    const uint32_t F = 1u << s.logf, G = guide_words(F);
     for (uint32_t j = threadIdx.x; j < F; j += NT) lsrt[j] = trees[s.tree_base + j];
     for (uint32_t j = threadIdx.x; j < G; j += NT) lguide[j] = guides[s.guide_base + j];
-    return Guide{s.kmin, s.shift, G - 1};
+    return make_guide(s.kmin, s.shift, F, s.flags);
 }
 
 // Bucket of key x in segment s (learned segments use only even buckets).
@@ -148,6 +188,88 @@ __device__ __forceinline__ uint32_t classify_seg(const PartSeg& s, const K* lsrt
         return 2u * (uint32_t)b;
     }
     return classify(lsrt, lguide, g, x);
+}
+
+// Buckets of NB keys of one segment at once.  Each stage issues its LDS reads
+// for all NB keys before any result is used and no stage branches per key, so
+// the reads of different keys overlap (one classification is a chain of three
+// to five dependent LDS reads).  Keys of direct top cells also run the
+// sub-table stage (on header 0, which the tree build always writes) and keep
+// their top entry.
+template <class K, int NB>
+__device__ __forceinline__ void classify_batch(const K* srt, const uint32_t* gd, const Guide& g, const K (&x)[NB],
+                                               uint32_t (&bk)[NB]) {
+    uint32_t e[NB];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+        const uint64_t xv = (uint64_t)x[u];
+        const uint64_t c = xv < g.kmin ? 0 : (xv - g.kmin) >> g.shift;
+        e[u] = gd[c > g.topmask ? g.topmask : (uint32_t)c];
+    }
+    uint32_t lo[NB], len[NB];
+    if (g.flags & GUIDE_SUBTABLES) {
+        uint4 h[NB];
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            const uint32_t hi = (e[u] & GUIDE_SUBF) ? (e[u] & 0xFFFFu) : 0u;
+            h[u] = *reinterpret_cast<const uint4*>(gd + g.hdr + 4 * hi);
+        }
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            const uint64_t xv = (uint64_t)x[u];
+            const uint64_t org = (uint64_t)h[u].x | ((uint64_t)h[u].y << 32);
+            const uint32_t m = (1u << (h[u].w >> 16)) - 1u;
+            const uint64_t q = xv < org ? 0 : (xv - org) >> h[u].z;
+            const uint32_t e2 = gd[g.sub + (h[u].w & 0xFFFFu) + (q > m ? m : (uint32_t)q)];
+            e[u] = (e[u] & GUIDE_SUBF) ? e2 : e[u];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+        lo[u] = e[u] & 0xFFFFu;
+        len[u] = ((e[u] >> 16) & 0x7FFFu) - lo[u] + 1;
+    }
+    // j is in [lo, lo + len) and srt[lo + len - 1] >= x: halve until one is left
+    for (;;) {
+        uint32_t mx = 0;
+#pragma unroll
+        for (int u = 0; u < NB; ++u) mx = len[u] > mx ? len[u] : mx;
+        if (mx <= 1) break;
+        K sv[NB];
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            const uint32_t half = len[u] >> 1;
+            sv[u] = srt[lo[u] + (half ? half - 1 : 0)];
+        }
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            const uint32_t half = len[u] >> 1;
+            lo[u] += (half && sv[u] < x[u]) ? half : 0u;
+            len[u] -= half;
+        }
+    }
+    if (g.flags & GUIDE_EQBUCKETS) {
+        K ev[NB];
+#pragma unroll
+        for (int u = 0; u < NB; ++u) ev[u] = srt[lo[u]];
+#pragma unroll
+        for (int u = 0; u < NB; ++u) bk[u] = 2 * lo[u] + (ev[u] == x[u] ? 1u : 0u);
+    } else {
+#pragma unroll
+        for (int u = 0; u < NB; ++u) bk[u] = 2 * lo[u];
+    }
+}
+
+template <class K, int NB>
+__device__ __forceinline__ void classify_seg_batch(const PartSeg& s, const K* lsrt, const uint32_t* lguide,
+                                                   const Guide& g, const LearnedTable& lt, const K (&x)[NB],
+                                                   uint32_t (&bk)[NB]) {
+    if (s.mode) {
+#pragma unroll
+        for (int u = 0; u < NB; ++u) bk[u] = classify_seg(s, lsrt, lguide, g, lt, x[u]);
+    } else {
+        classify_batch<K, NB>(lsrt, lguide, g, x, bk);
+    }
 }
 
 // ---------------------------------------------------------------- tile kernels
@@ -336,15 +458,34 @@ __global__ __launch_bounds__(256) void k_sample(const PartSeg* ps, const uint32_
 }
 
 // Splitters at evenly spaced ranks of the sorted sample, duplicates removed,
-// padded with KMAX to F entries; then the guide table over [kmin, kmax] of
-// the sample.  One workgroup per segment.  Writes kmin/shift into the
-// segment descriptor for the count and scatter kernels.
+// padded with KMAX to F entries; then the two-level guide over [kmin, kmax]
+// of the sample.  One workgroup of FMAX threads per segment.  Writes
+// kmin/shift into the segment descriptor for the count and scatter kernels.
+struct TreeLds {
+    uint32_t wtot[FMAX / 64];
+    uint32_t hbase[guide_hdrs(FMAX)];  // sub-table base | log2(cells) << 16
+    uint32_t hj[guide_hdrs(FMAX)];     // j0 | j1 << 16 of the dense top cell
+    uint32_t hsh[guide_hdrs(FMAX)];    // shift2
+    uint64_t horg[guide_hdrs(FMAX)];   // origin (encoded key)
+    uint32_t nuniq, nhdr;
+};
+
+__device__ __forceinline__ uint64_t sat_add_shl(uint64_t org, uint64_t q, uint32_t sh, uint64_t kmax) {
+    // min(kmax, org + (q << sh)) without overflow
+    if (q > ((kmax - org) >> sh)) return kmax;
+    const uint64_t off = q << sh;
+    return off > kmax - org ? kmax : org + off;
+}
+
+__device__ __forceinline__ uint32_t pow2ceil32(uint32_t v) { return v <= 1 ? 1u : 1u << (32 - __clz(v - 1)); }
+
 template <class K>
 __device__ __forceinline__ void build_tree_body(PartSeg& s, const K* smp, K* trees, uint32_t* guides, K* srt,
-                                                uint32_t* wtot) {
-    const uint32_t F = 1u << s.logf, G = GUIDE_PER_SPLIT * F;
+                                                TreeLds& L) {
+    const uint32_t F = 1u << s.logf;
     const uint32_t ns = s.nsample;
     const uint32_t t = threadIdx.x;
+    const uint64_t KM = (uint64_t)(K)~(K)0;
     K c = (K)~(K)0;
     bool keep = false;
     if (t + 1 < F) {
@@ -357,30 +498,97 @@ __device__ __forceinline__ void build_tree_body(PartSeg& s, const K* smp, K* tre
         }
     }
     uint32_t nuniq;
-    const uint32_t pos = block_excl_scan<FMAX>(keep ? 1u : 0u, wtot, &nuniq);
+    const uint32_t pos = block_excl_scan<FMAX>(keep ? 1u : 0u, L.wtot, &nuniq);
     if (t < F) srt[t] = (K)~(K)0;
     __syncthreads();
     if (keep) srt[pos] = c;
     __syncthreads();
     K* out = trees + s.tree_base;
     if (t < F) out[t] = srt[t];
-    // guide table over the sample's key range
+    // top cells over the sample's key range
     const uint64_t kmin = (uint64_t)smp[0], kmax = (uint64_t)smp[ns - 1];
-    const uint64_t range = kmax - kmin;
+    const uint32_t TOP = guide_top(F);
     uint32_t shift = 0;
-    while ((range >> shift) >= G) ++shift;
-    for (uint32_t cc = t; cc < G; cc += FMAX) {
-        const uint64_t off0 = (uint64_t)cc << shift, off1 = (uint64_t)(cc + 1) << shift;
-        const uint64_t kmaxk = (uint64_t)(K)~(K)0;
-        const K lo_key = (K)(off0 > kmaxk - kmin ? kmaxk : kmin + off0);
-        const K hi_key = (K)(off1 > kmaxk - kmin ? kmaxk : kmin + off1);
-        const uint32_t j0 = cc == 0 ? 0u : lower_bound_lds(srt, F - 1, lo_key);
-        const uint32_t j1 = cc == G - 1 ? F - 1 : lower_bound_lds(srt, F - 1, hi_key);
-        guides[s.guide_base + cc] = j0 | (j1 << 16);
+    while (((kmax - kmin) >> shift) >= TOP) ++shift;
+    uint32_t* gd = guides + s.guide_base;
+    const Guide g = make_guide(kmin, shift, F, 0);
+    // each thread computes CPT consecutive top cells (TOP <= 4 * FMAX)
+    constexpr uint32_t CPT_MAX = GUIDE_TOP_MAX / FMAX;
+    static_assert(CPT_MAX * FMAX == GUIDE_TOP_MAX, "top cells per thread");
+    const uint32_t CPT = TOP > FMAX ? TOP / FMAX : 1u;
+    uint32_t cj0[CPT_MAX], cj1[CPT_MAX], csc[CPT_MAX];
+    uint32_t my_dense = 0, my_sc = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < CPT_MAX; ++k) {
+        const uint32_t c = t * CPT + k;
+        cj0[k] = cj1[k] = csc[k] = 0;
+        if (k < CPT && c < TOP) {
+            const uint64_t lo_key = sat_add_shl(kmin, c, shift, KM);
+            const uint64_t hi_key = sat_add_shl(kmin, c + 1, shift, KM);
+            cj0[k] = c == 0 ? 0u : lower_bound_lds(srt, nuniq, (K)lo_key);
+            cj1[k] = c == TOP - 1 ? nuniq : lower_bound_lds(srt, nuniq, (K)hi_key);
+            if (cj1[k] - cj0[k] > GUIDE_DENSE) {
+                csc[k] = pow2ceil32(4 * (cj1[k] - cj0[k]));
+                my_dense += 1;
+                my_sc += csc[k];
+            }
+        }
+    }
+    uint32_t nhdr, nsub;
+    uint32_t h = block_excl_scan<FMAX>(my_dense, L.wtot, &nhdr);
+    uint32_t sbase = block_excl_scan<FMAX>(my_sc, L.wtot, &nsub);
+#pragma unroll
+    for (uint32_t k = 0; k < CPT_MAX; ++k) {
+        const uint32_t c = t * CPT + k;
+        if (!(k < CPT && c < TOP)) continue;
+        const uint32_t j0 = cj0[k], j1 = cj1[k], sc = csc[k];
+        if (sc) {
+            const uint64_t org = (uint64_t)srt[j0], last = (uint64_t)srt[j1 - 1];
+            uint32_t sh2 = 0;
+            while (((last - org) >> sh2) >= sc) ++sh2;
+            const uint32_t lg = 31 - __clz(sc);
+            L.hbase[h] = sbase | (lg << 16);
+            L.hj[h] = j0 | (j1 << 16);
+            L.hsh[h] = sh2;
+            L.horg[h] = org;
+            gd[c] = GUIDE_SUBF | h;
+            gd[g.hdr + 4 * h + 0] = (uint32_t)org;
+            gd[g.hdr + 4 * h + 1] = (uint32_t)(org >> 32);
+            gd[g.hdr + 4 * h + 2] = sh2;
+            gd[g.hdr + 4 * h + 3] = sbase | (lg << 16);
+            h += 1;
+            sbase += sc;
+        } else {
+            gd[c] = j0 | (j1 << 16);
+        }
+    }
+    // unused headers are zero: classify_batch reads header 0 for every key
+    for (uint32_t q = nhdr + t; q < guide_hdrs(F); q += FMAX) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) gd[g.hdr + 4 * q + k] = 0u;
+    }
+    __syncthreads();
+    // sub-table cells: q -> its header by binary search over the bases
+    for (uint32_t q = t; q < nsub; q += FMAX) {
+        uint32_t lo = 0, hi = nhdr;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if ((L.hbase[mid] & 0xFFFFu) <= q) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t hb = L.hbase[lo], hjj = L.hj[lo], sh2 = L.hsh[lo];
+        const uint64_t org = L.horg[lo];
+        const uint32_t loc = q - (hb & 0xFFFFu), cells = 1u << (hb >> 16);
+        const uint32_t a0 = hjj & 0xFFFFu, a1 = hjj >> 16;
+        const uint32_t jl = loc == 0 ? a0 : a0 + lower_bound_lds(srt + a0, a1 - a0, (K)sat_add_shl(org, loc, sh2, KM));
+        const uint32_t jh =
+            loc == cells - 1 ? a1 : a0 + lower_bound_lds(srt + a0, a1 - a0, (K)sat_add_shl(org, loc + 1, sh2, KM));
+        gd[g.sub + q] = jl | (jh << 16);
     }
     if (t == 0) {
         s.kmin = kmin;
         s.shift = shift;
+        s.flags = (nhdr ? GUIDE_SUBTABLES : 0u) | (nuniq + 1 < F ? GUIDE_EQBUCKETS : 0u);
     }
 }
 
@@ -388,11 +596,11 @@ template <class K>
 __global__ __launch_bounds__(FMAX) void k_build_tree(PartSeg* ps, const uint32_t* nseg, const K* samples,
                                                      uint32_t smax, K* trees, uint32_t* guides) {
     __shared__ K srt[FMAX];
-    __shared__ uint32_t wtot[FMAX / 64];
+    __shared__ TreeLds L;
     for (uint32_t w = blockIdx.x; w < *nseg; w += gridDim.x) {
         PartSeg& s = ps[w];
         if (s.mode || s.nsample == 0) continue;  // learned segment: no splitters
-        build_tree_body<K>(s, samples + s.sample_base, trees, guides, srt, wtot);
+        build_tree_body<K>(s, samples + s.sample_base, trees, guides, srt, L);
         __syncthreads();
     }
 }
@@ -404,16 +612,61 @@ __device__ __forceinline__ bool nan_or_negzero(uint64_t k) {
 }
 __device__ __forceinline__ bool nan_or_negzero(uint32_t) { return false; }
 
-template <class T>
+// Histogram and rank helpers over the E keys a thread holds (all lanes of the
+// wave active).  Every lane adds one to its key's bucket.  At a key position
+// where the whole wave holds one bucket (sorted or run-heavy input) the first
+// lane adds the wave's count and the other lanes add zero to a private dummy
+// slot, so 64 lanes do not serialise on one LDS word.  No branch depends on
+// the keys, and all atomics are issued before any return value is used.
+constexpr int DUMMY_SLOTS = 64;
+
+template <int E>
+__device__ __forceinline__ void wave_hist_add_many(uint32_t* cnt, uint32_t* dummy, const uint32_t (&bk)[E]) {
+    const uint32_t lane = __lane_id();
+    const bool first = lane == 0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const uint32_t b0 = __builtin_amdgcn_readfirstlane(bk[e]);
+        const bool same = __ballot(bk[e] == b0) == ~0ull;
+        uint32_t* addr = same ? (first ? &cnt[b0] : &dummy[lane]) : &cnt[bk[e]];
+        atomicAdd(addr, same ? (first ? 64u : 0u) : 1u);
+    }
+}
+
+template <int E>
+__device__ __forceinline__ void wave_rank_many(uint32_t* cnt, uint32_t* dummy, const uint32_t (&bk)[E],
+                                               uint32_t (&lr)[E]) {
+    const uint32_t lane = __lane_id();
+    const bool first = lane == 0;
+    bool same[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const uint32_t b0 = __builtin_amdgcn_readfirstlane(bk[e]);
+        same[e] = __ballot(bk[e] == b0) == ~0ull;
+        uint32_t* addr = same[e] ? (first ? &cnt[b0] : &dummy[lane]) : &cnt[bk[e]];
+        lr[e] = atomicAdd(addr, same[e] ? (first ? 64u : 0u) : 1u);
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) lr[e] = same[e] ? __builtin_amdgcn_readfirstlane(lr[e]) + lane : lr[e];
+}
+
+// Per chunk: bucket histogram -> hist[chunk][bucket].  With ids != nullptr
+// the bucket of every key is also stored (ids[position] = bucket), so the
+// scatter does not classify again.  Batches of CNT_P*CNT_U keys alternate
+// between two register buffers: the next batch is in flight while this one is
+// classified, with no register copies that would wait for it.
+template <class T, int CNT_P, int CNT_U, int NB>
 __global__ __launch_bounds__(CNT_P) void k_count(const PartSeg* ps, const ChunkRef* cr, const uint32_t* nchunk,
                                                  const T* a, const T* b, const typename KeyTraits<T>::K* trees,
                                                  const uint32_t* guides, uint32_t* hist, uint32_t* special,
-                                                 LearnedTable lt) {
+                                                 LearnedTable lt, uint16_t* ids) {
     using Tr = KeyTraits<T>;
     using K = typename Tr::K;
+    constexpr int STEP = CNT_P * CNT_U;
     __shared__ K lsrt[FMAX];
-    __shared__ uint32_t lguide[LDS_TABLE_WORDS];
+    __shared__ __align__(16) uint32_t lguide[LDS_TABLE_WORDS];
     __shared__ uint32_t lh[BMAX];
+    __shared__ uint32_t dummy[DUMMY_SLOTS];
     for (uint32_t w = blockIdx.x; w < *nchunk; w += gridDim.x) {
     const ChunkRef c = cr[w];
     const PartSeg s = ps[c.seg];
@@ -423,26 +676,59 @@ __global__ __launch_bounds__(CNT_P) void k_count(const PartSeg* ps, const ChunkR
     for (uint32_t j = t; j < B; j += CNT_P) lh[j] = 0;
     __syncthreads();
     const T* src = (s.buf ? b : a) + s.begin;
+    uint16_t* oid = ids ? ids + s.begin : nullptr;
     const uint64_t lo = (uint64_t)c.idx * s.chunk;
     const uint64_t hi = lo + s.chunk < s.size ? lo + s.chunk : s.size;
     bool odd = false;
-    for (uint64_t base = lo; base < hi; base += (uint64_t)CNT_P * CNT_U) {
-        // clamped indices keep every load in bounds and unconditional, so all
-        // CNT_U loads of a thread are in flight together
-        K x[CNT_U];
+    // clamped indices keep every load in bounds and unconditional
+    auto load = [&](uint64_t base, K (&d)[CNT_U]) {
 #pragma unroll
         for (int u = 0; u < CNT_U; ++u) {
             const uint64_t i = base + (uint64_t)u * CNT_P + t;
-            x[u] = Tr::enc(src[i < hi ? i : hi - 1]);
+            d[u] = Tr::enc(src[i < hi ? i : hi - 1]);
         }
+    };
+    auto batch = [&](uint64_t base, K (&x)[CNT_U], K (&xn)[CNT_U]) {
+        if (base + STEP < hi) load(base + STEP, xn);
+        if (base + STEP <= hi) {
+            uint32_t bk[CNT_U];
 #pragma unroll
-        for (int u = 0; u < CNT_U; ++u) {
-            const uint64_t i = base + (uint64_t)u * CNT_P + t;
-            if (i < hi) {
+            for (int u0 = 0; u0 < CNT_U; u0 += NB) {
+                K xb[NB];
+                uint32_t kb[NB];
+#pragma unroll
+                for (int u = 0; u < NB; ++u) xb[u] = x[u0 + u];
+                classify_seg_batch<K, NB>(s, lsrt, lguide, g, lt, xb, kb);
+#pragma unroll
+                for (int u = 0; u < NB; ++u) bk[u0 + u] = kb[u];
+            }
+            wave_hist_add_many<CNT_U>(lh, dummy, bk);
+#pragma unroll
+            for (int u = 0; u < CNT_U; ++u) {
                 odd |= nan_or_negzero(x[u]);
-                atomicAdd(&lh[classify_seg(s, lsrt, lguide, g, lt, x[u])], 1u);
+                if (oid) oid[base + (uint64_t)u * CNT_P + t] = (uint16_t)bk[u];
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < CNT_U; ++u) {
+                const uint64_t i = base + (uint64_t)u * CNT_P + t;
+                if (i < hi) {
+                    const uint32_t bk = classify_seg(s, lsrt, lguide, g, lt, x[u]);
+                    odd |= nan_or_negzero(x[u]);
+                    atomicAdd(&lh[bk], 1u);
+                    if (oid) oid[i] = (uint16_t)bk;
+                }
             }
         }
+    };
+    K xa[CNT_U], xb[CNT_U];
+    if (lo < hi) load(lo, xa);
+    for (uint64_t base = lo; base < hi;) {
+        batch(base, xa, xb);
+        base += STEP;
+        if (base >= hi) break;
+        batch(base, xb, xa);
+        base += STEP;
     }
     if (__any(odd) && (t & 63u) == 0) atomicOr(special, 1u);
     __syncthreads();
@@ -452,169 +738,304 @@ __global__ __launch_bounds__(CNT_P) void k_count(const PartSeg* ps, const ChunkR
     }
 }
 
+// Count variants: workgroup threads, keys per thread per batch, and keys
+// classified together.  grid = chunks (the kernel loops over the device count).
+template <class T>
+inline void launch_count(int shape, uint32_t grid, hipStream_t st, const PartSeg* ps, const ChunkRef* cr,
+                         const uint32_t* nchunk, const T* a, const T* b, const typename KeyTraits<T>::K* trees,
+                         const uint32_t* guides, uint32_t* hist, uint32_t* special, const LearnedTable& lt,
+                         uint16_t* ids) {
+#define S512_COUNT(PP, UU, NN) \
+    k_count<T, PP, UU, NN><<<grid, PP, 0, st>>>(ps, cr, nchunk, a, b, trees, guides, hist, special, lt, ids)
+    switch (shape) {
+        case 1: S512_COUNT(256, 16, 4); break;
+        case 2: S512_COUNT(256, 8, 8); break;
+        case 3: S512_COUNT(512, 8, 8); break;
+        case 4: S512_COUNT(256, 8, 4); break;
+        default: S512_COUNT(256, 16, 8); break;
+    }
+#undef S512_COUNT
+}
+
 // ---------------------------------------------------------------- column scan
-// hist[chunk][bucket] -> prefix over earlier chunks of the same bucket;
-// bstart[b] = first position of bucket b inside the segment; bsize[b] = size.
-[[maybe_unused]] static __global__ __launch_bounds__(BMAX) void k_colscan(const PartSeg* ps, const uint32_t* nseg,
-                                                                         uint32_t* hist, uint64_t* bstart,
-                                                                         uint64_t* bsize) {
+// hist[chunk][bucket] of every segment -> prefix over earlier chunks of the
+// same bucket (in place); bsize[b] = bucket size.  One workgroup per
+// (segment, slice of 64 buckets): lane = bucket, wave = a quarter of the
+// chunks; loads are batched so a column of ~1000 chunks is not a chain of
+// dependent round trips.
+constexpr int CS_SLICE = 64;
+constexpr int CS_NSLICE = BMAX / CS_SLICE;
+constexpr int CS_BATCH = 8;
+[[maybe_unused]] static __global__ __launch_bounds__(256) void k_colscan(const PartSeg* ps, const uint32_t* nseg,
+                                                                        uint32_t* hist, uint64_t* bsize) {
+    __shared__ uint32_t part[4][CS_SLICE];
+    const uint32_t lane = threadIdx.x & 63u, grp = threadIdx.x >> 6;
+    const uint32_t nwork = *nseg * CS_NSLICE;
+    for (uint32_t w = blockIdx.x; w < nwork; w += gridDim.x) {
+        const PartSeg s = ps[w / CS_NSLICE];
+        const uint32_t B = 2u << s.logf;
+        const uint32_t b0 = (w % CS_NSLICE) * CS_SLICE;
+        if (b0 >= B) continue;  // whole workgroup: no barrier is skipped by part of it
+        const uint32_t bkt = b0 + lane;
+        const bool on = bkt < B;
+        const uint32_t nc = s.nchunks;
+        const uint32_t c0 = (uint32_t)((uint64_t)nc * grp / 4), c1 = (uint32_t)((uint64_t)nc * (grp + 1) / 4);
+        uint32_t* h = hist + s.hist_base + bkt;
+        uint32_t sum = 0;
+        for (uint32_t cb = c0; cb < c1; cb += CS_BATCH) {
+            uint32_t v[CS_BATCH];
+#pragma unroll
+            for (int k = 0; k < CS_BATCH; ++k) v[k] = (on && cb + k < c1) ? h[(uint64_t)(cb + k) * B] : 0u;
+#pragma unroll
+            for (int k = 0; k < CS_BATCH; ++k) sum += v[k];
+        }
+        part[grp][lane] = sum;
+        __syncthreads();
+        uint32_t run = 0, total = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t p = part[q][lane];
+            run += q < (int)grp ? p : 0u;
+            total += p;
+        }
+        for (uint32_t cb = c0; cb < c1; cb += CS_BATCH) {
+            uint32_t v[CS_BATCH];
+#pragma unroll
+            for (int k = 0; k < CS_BATCH; ++k) v[k] = (on && cb + k < c1) ? h[(uint64_t)(cb + k) * B] : 0u;
+#pragma unroll
+            for (int k = 0; k < CS_BATCH; ++k) {
+                if (on && cb + k < c1) h[(uint64_t)(cb + k) * B] = run;
+                run += v[k];
+            }
+        }
+        if (on && grp == 0) bsize[s.bkt_base + bkt] = total;
+        __syncthreads();
+    }
+}
+
+// bstart[b] = first position of bucket b inside its segment.
+[[maybe_unused]] static __global__ __launch_bounds__(BMAX) void k_bucket_scan(const PartSeg* ps, const uint32_t* nseg,
+                                                                             const uint64_t* bsize, uint64_t* bstart) {
     __shared__ uint32_t wtot[BMAX / 64];
     for (uint32_t w = blockIdx.x; w < *nseg; w += gridDim.x) {
-    const PartSeg s = ps[w];
-    const uint32_t B = 2u << s.logf;
-    const uint32_t t = threadIdx.x;
-    uint32_t run = 0;
-    if (t < B) {
-        uint32_t* h = hist + s.hist_base + t;
-        for (uint32_t c = 0; c < s.nchunks; ++c) {
-            const uint32_t v = h[(uint64_t)c * B];
-            h[(uint64_t)c * B] = run;
-            run += v;
-        }
-    }
-    uint32_t total;
-    const uint32_t start = block_excl_scan<BMAX>(run, wtot, &total);
-    if (t < B) {
-        bstart[s.bkt_base + t] = start;
-        bsize[s.bkt_base + t] = run;
-    }
+        const PartSeg s = ps[w];
+        const uint32_t B = 2u << s.logf;
+        const uint32_t t = threadIdx.x;
+        const uint32_t v = t < B ? (uint32_t)bsize[s.bkt_base + t] : 0u;
+        uint32_t total;
+        const uint32_t start = block_excl_scan<BMAX>(v, wtot, &total);
+        if (t < B) bstart[s.bkt_base + t] = start;
     }
 }
 
 // ---------------------------------------------------------------- scatter
 // Keys (and values for key/value sorts) move from array buf to array 1-buf.
-// Per round of SCAT_ROUND keys:
-//   classify, rank inside the round with LDS atomics           | barrier 1
-//   wave-level scan of the round histogram (2 buckets/thread)  | barrier 2
-//   bucket owners: round start rs, destination base, advance the
-//   chunk's cursor, clear the counter for the next round       | barrier 3
-//   stage keys grouped by bucket in LDS                        | barrier 4
+// A workgroup of P threads walks its chunk in rounds of P*E keys:
+//   bucket of each key (classified here, or read from the ids the count
+//   pass stored), rank inside the round with LDS atomics     | barrier 1
+//   wave-level scan of the round histogram (BMAX/P buckets per thread)
+//                                                            | barrier 2
+//   bucket owners: round start rs, destination base, advance the chunk's
+//   cursor, clear the counter for the next round             | barrier 3
+//   stage keys grouped by bucket in LDS                      | barrier 4
 //   write the staged keys: consecutive threads, consecutive addresses
 // The next round's keys are loaded into registers while this round runs.
-template <class T, bool KV>
-__global__ __launch_bounds__(SCAT_P) void k_scatter(const PartSeg* ps, const ChunkRef* cr, T* a, T* b,
-                                                    int32_t* va, int32_t* vb,
-                                                    const typename KeyTraits<T>::K* trees, const uint32_t* guides,
-                                                    const uint32_t* hist, const uint64_t* bstart, LearnedTable lt,
-                                                    const uint32_t* nchunk) {
+template <class T, bool KV, bool IDS, int P, int E>
+struct ScatterLds {
+    using K = typename KeyTraits<T>::K;
+    static constexpr int ROUND = P * E;
+    static constexpr bool CLS = !IDS;  // classifier tables needed
+    K lsrt[CLS ? FMAX : 1];
+    __align__(16) uint32_t lguide[CLS ? LDS_TABLE_WORDS : 1];
+    uint32_t cursor[BMAX];  // next destination of each bucket (segment-relative)
+    uint32_t rh[BMAX];      // round histogram
+    uint16_t rs[BMAX];      // round start of each bucket in the staging area
+    uint32_t dbase[BMAX];   // destination of staged slot i is dbase[bucket] + i
+    uint32_t wtot[P / 64];
+    uint32_t dummy[DUMMY_SLOTS];
+    T stg[ROUND];
+    uint16_t stb[ROUND];
+    int32_t stv[KV ? ROUND : 1];
+};
+
+template <class T, bool KV, bool IDS, int P, int E>
+__global__ __launch_bounds__(P) void k_scatter(const PartSeg* ps, const ChunkRef* cr, T* a, T* b, int32_t* va,
+                                               int32_t* vb, const typename KeyTraits<T>::K* trees,
+                                               const uint32_t* guides, const uint32_t* hist, const uint64_t* bstart,
+                                               LearnedTable lt, const uint32_t* nchunk, const uint16_t* ids) {
     using Tr = KeyTraits<T>;
     using K = typename Tr::K;
-    constexpr int NW = SCAT_P / 64;
-    static_assert(2 * SCAT_P == BMAX, "scan: two buckets per thread");
-    __shared__ K lsrt[FMAX];
-    __shared__ uint32_t lguide[LDS_TABLE_WORDS];
-    __shared__ uint32_t cursor[BMAX];  // next destination of each bucket (segment-relative)
-    __shared__ uint32_t rh[BMAX];      // round histogram
-    __shared__ uint32_t rs[BMAX];      // round start of each bucket in the staging area
-    __shared__ uint32_t dbase[BMAX];   // destination of staged slot i is dbase[bucket] + i
-    __shared__ uint32_t wtot[NW];
-    __shared__ T stg[SCAT_ROUND];
-    __shared__ uint16_t stb[SCAT_ROUND];
-    __shared__ int32_t stv[KV ? SCAT_ROUND : 1];
+    using L = ScatterLds<T, KV, IDS, P, E>;
+    constexpr int NW = P / 64;
+    constexpr int ROUND = L::ROUND;
+    constexpr int BPT = BMAX / P;  // buckets per thread in the scan
+    static_assert(BPT * P == BMAX && ROUND <= 32768, "scatter shape");
+    __shared__ L sm;
 
     for (uint32_t wi = blockIdx.x; wi < *nchunk; wi += gridDim.x) {
     const ChunkRef c = cr[wi];
     const PartSeg s = ps[c.seg];
-    const uint32_t F = 1u << s.logf, B = 2 * F;
+    const uint32_t B = 2u << s.logf;
     const uint32_t t = threadIdx.x, lane = t & 63u, wid = t >> 6;
-    const Guide g = load_classifier<K, SCAT_P>(s, trees, guides, lsrt, lguide, lt);
+    Guide g{0, 0, 0, 0, 0, 0};
+    if constexpr (!IDS) g = load_classifier<K, P>(s, trees, guides, sm.lsrt, sm.lguide, lt);
     const uint32_t* h = hist + s.hist_base + (uint64_t)c.idx * B;
-    for (uint32_t j = t; j < BMAX; j += SCAT_P) {
-        cursor[j] = j < B ? (uint32_t)bstart[s.bkt_base + j] + h[j] : 0u;
-        rh[j] = 0;
+    for (uint32_t j = t; j < BMAX; j += P) {
+        sm.cursor[j] = j < B ? (uint32_t)bstart[s.bkt_base + j] + h[j] : 0u;
+        sm.rh[j] = 0;
     }
     const T* src = (s.buf ? b : a) + s.begin;
     T* dst = (s.buf ? a : b) + s.begin;
     const int32_t* vsrc = KV ? ((s.buf ? vb : va) + s.begin) : nullptr;
     int32_t* vdst = KV ? ((s.buf ? va : vb) + s.begin) : nullptr;
+    const uint16_t* isrc = IDS ? ids + s.begin : nullptr;
     const uint64_t lo = (uint64_t)c.idx * s.chunk;
     const uint64_t hi = lo + s.chunk < s.size ? lo + s.chunk : s.size;
 
-    T x[SCAT_E], xn[SCAT_E];
-    int32_t xv[SCAT_E], xvn[SCAT_E];
-    auto load_round = [&](uint64_t base, T (&dx)[SCAT_E], int32_t (&dv)[SCAT_E]) {
-        const uint32_t cnt = (uint32_t)((hi - base) < (uint64_t)SCAT_ROUND ? (hi - base) : (uint64_t)SCAT_ROUND);
+    // rounds alternate between two register buffers: the next round's keys
+    // are in flight while this one runs, with no copies that would wait for them
+    struct Buf {
+        T x[E];
+        int32_t v[E];
+        uint32_t b[E];
+    };
+    auto load_round = [&](uint64_t base, Buf& R) {
+        const uint32_t cnt = (uint32_t)((hi - base) < (uint64_t)ROUND ? (hi - base) : (uint64_t)ROUND);
 #pragma unroll
-        for (int e = 0; e < SCAT_E; ++e) {
-            const uint32_t i = e * SCAT_P + t;
+        for (int e = 0; e < E; ++e) {
+            const uint32_t i = e * P + t;
             const uint64_t ic = base + (i < cnt ? i : cnt - 1);  // in bounds, unconditional
-            dx[e] = src[ic];
-            if constexpr (KV) dv[e] = vsrc[ic];
+            R.x[e] = src[ic];
+            if constexpr (KV) R.v[e] = vsrc[ic];
+            if constexpr (IDS) R.b[e] = isrc[ic];
         }
     };
-    if (lo < hi) load_round(lo, x, xv);
-    __syncthreads();
-
-    for (uint64_t base = lo; base < hi; base += SCAT_ROUND) {
-        const uint32_t cnt = (uint32_t)((hi - base) < (uint64_t)SCAT_ROUND ? (hi - base) : (uint64_t)SCAT_ROUND);
-        if (base + SCAT_ROUND < hi) load_round(base + SCAT_ROUND, xn, xvn);  // prefetch
-        uint32_t bk[SCAT_E], lr[SCAT_E];
+    auto do_round = [&](uint64_t base, Buf& cur, Buf& nxt) {
+        const uint32_t cnt = (uint32_t)((hi - base) < (uint64_t)ROUND ? (hi - base) : (uint64_t)ROUND);
+        if (base + ROUND < hi) load_round(base + ROUND, nxt);  // prefetch
+        uint32_t bk[E], lr[E];
+        if constexpr (IDS) {
 #pragma unroll
-        for (int e = 0; e < SCAT_E; ++e) {
-            const uint32_t i = e * SCAT_P + t;
-            bk[e] = classify_seg(s, lsrt, lguide, g, lt, Tr::enc(x[e]));
-            if (i < cnt) lr[e] = atomicAdd(&rh[bk[e]], 1u);
+            for (int e = 0; e < E; ++e) bk[e] = cur.b[e];
+        } else if (cnt == ROUND) {
+#pragma unroll
+            for (int e0 = 0; e0 < E; e0 += CLS_NB) {
+                K kx[CLS_NB];
+                uint32_t kb[CLS_NB];
+#pragma unroll
+                for (int u = 0; u < CLS_NB; ++u) kx[u] = Tr::enc(cur.x[e0 + u]);
+                classify_seg_batch<K, CLS_NB>(s, sm.lsrt, sm.lguide, g, lt, kx, kb);
+#pragma unroll
+                for (int u = 0; u < CLS_NB; ++u) bk[e0 + u] = kb[u];
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < E; ++e) bk[e] = classify_seg(s, sm.lsrt, sm.lguide, g, lt, Tr::enc(cur.x[e]));
+        }
+        if (cnt == ROUND) {
+            wave_rank_many<E>(sm.rh, sm.dummy, bk, lr);
+        } else {
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+                if (e * P + t < cnt) lr[e] = atomicAdd(&sm.rh[bk[e]], 1u);
         }
         __syncthreads();  // 1
-        // exclusive scan of rh: thread t owns buckets 2t, 2t+1
-        const uint32_t j0 = 2 * t, j1 = 2 * t + 1;
-        const uint32_t h0 = rh[j0], h1 = rh[j1];
-        uint32_t v = h0 + h1, incl = v;
+        // exclusive scan of rh: thread t owns buckets t*BPT .. t*BPT+BPT-1
+        uint32_t hv[BPT], v = 0;
+#pragma unroll
+        for (int k = 0; k < BPT; ++k) {
+            hv[k] = sm.rh[t * BPT + k];
+            v += hv[k];
+        }
+        uint32_t incl = v;
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
             const uint32_t y = __shfl_up(incl, d);
             if (lane >= (uint32_t)d) incl += y;
         }
-        if (lane == 63) wtot[wid] = incl;
+        if (lane == 63) sm.wtot[wid] = incl;
         __syncthreads();  // 2
         uint32_t woff = 0;
 #pragma unroll
-        for (int w = 0; w < NW; ++w) woff += (w < (int)wid) ? wtot[w] : 0u;
-        const uint32_t p0 = woff + incl - v, p1 = p0 + h0;
-        rs[j0] = p0;
-        rs[j1] = p1;
-        dbase[j0] = cursor[j0] - p0;
-        dbase[j1] = cursor[j1] - p1;
-        cursor[j0] += h0;
-        cursor[j1] += h1;
-        rh[j0] = 0;
-        rh[j1] = 0;
+        for (int w = 0; w < NW; ++w) woff += (w < (int)wid) ? sm.wtot[w] : 0u;
+        uint32_t p = woff + incl - v;
+#pragma unroll
+        for (int k = 0; k < BPT; ++k) {
+            const uint32_t j = t * BPT + k;
+            sm.rs[j] = (uint16_t)p;
+            sm.dbase[j] = sm.cursor[j] - p;
+            sm.cursor[j] += hv[k];
+            sm.rh[j] = 0;
+            p += hv[k];
+        }
         __syncthreads();  // 3
 #pragma unroll
-        for (int e = 0; e < SCAT_E; ++e) {
-            const uint32_t i = e * SCAT_P + t;
+        for (int e = 0; e < E; ++e) {
+            const uint32_t i = e * P + t;
             if (i < cnt) {
-                const uint32_t p = rs[bk[e]] + lr[e];
-                stg[p] = x[e];
-                stb[p] = (uint16_t)bk[e];
-                if constexpr (KV) stv[p] = xv[e];
+                const uint32_t q = sm.rs[bk[e]] + lr[e];
+                sm.stg[q] = cur.x[e];
+                sm.stb[q] = (uint16_t)bk[e];
+                if constexpr (KV) sm.stv[q] = cur.v[e];
             }
         }
         __syncthreads();  // 4
 #pragma unroll
-        for (int e = 0; e < SCAT_E; ++e) {
-            const uint32_t i = e * SCAT_P + t;  // staged keys are grouped by bucket
-            const uint32_t bkt = stb[i] & (BMAX - 1u);  // slots >= cnt: stale, never stored
-            const uint64_t d = (uint64_t)dbase[bkt] + i;
-            const T y = stg[i];
+        for (int e = 0; e < E; ++e) {
+            const uint32_t i = e * P + t;  // staged keys are grouped by bucket
+            const uint32_t bkt = sm.stb[i] & (BMAX - 1u);  // slots >= cnt: stale, never stored
+            const uint64_t d = (uint64_t)sm.dbase[bkt] + i;
+            const T y = sm.stg[i];
             int32_t yv = 0;
-            if constexpr (KV) yv = stv[i];
+            if constexpr (KV) yv = sm.stv[i];
             if (i < cnt) {
                 dst[d] = y;
                 if constexpr (KV) vdst[d] = yv;
             }
         }
-#pragma unroll
-        for (int e = 0; e < SCAT_E; ++e) {
-            x[e] = xn[e];
-            if constexpr (KV) xv[e] = xvn[e];
-        }
         // the next round's atomics touch rh (cleared before barrier 3) and the
         // staging area is rewritten only after its barrier 3, which every
         // thread reaches after finishing the stores above
+    };
+    Buf A, Bb;
+    if (lo < hi) load_round(lo, A);
+    __syncthreads();
+    for (uint64_t base = lo; base < hi;) {
+        do_round(base, A, Bb);
+        base += ROUND;
+        if (base >= hi) break;
+        do_round(base, Bb, A);
+        base += ROUND;
     }
     __syncthreads();  // LDS is reloaded for the next chunk
     }
+}
+
+// Scatter variants: the workgroup shape, and whether the bucket of each key
+// is read from the ids the count pass stored or classified again.
+template <class T, bool KV>
+inline void launch_scatter(int shape, bool use_ids, uint32_t grid, hipStream_t st, const PartSeg* ps,
+                           const ChunkRef* cr, T* a, T* b, int32_t* va, int32_t* vb,
+                           const typename KeyTraits<T>::K* trees, const uint32_t* guides, const uint32_t* hist,
+                           const uint64_t* bstart, const LearnedTable& lt, const uint32_t* nchunk,
+                           const uint16_t* ids) {
+#define S512_SCATTER(I, PP, EE)                                                                               \
+    k_scatter<T, KV, I, PP, EE><<<grid, PP, 0, st>>>(ps, cr, a, b, va, vb, trees, guides, hist, bstart, lt, nchunk, \
+                                                     ids)
+    if (use_ids) {
+        switch (shape) {
+            case 1: S512_SCATTER(true, 256, 16); break;
+            case 2: S512_SCATTER(true, 1024, 8); break;
+            default: S512_SCATTER(true, 512, 16); break;
+        }
+    } else {
+        switch (shape) {
+            case 1: S512_SCATTER(false, 256, 16); break;
+            case 2: S512_SCATTER(false, 1024, 8); break;
+            default: S512_SCATTER(false, 512, 16); break;
+        }
+    }
+#undef S512_SCATTER
 }
 
 // ---------------------------------------------------------------- plan

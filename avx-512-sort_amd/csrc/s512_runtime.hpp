@@ -82,6 +82,7 @@ struct DeviceState {
     DevBuf arena;     // per-level metadata, histograms, lists
     DevBuf stage;     // caller-host-pointer staging
     DevBuf learn;     // learned partitioner: sample, training state, table
+    DevBuf ids;       // bucket of every key of a partition level (u16)
     HostBuf pinned;   // host<->device metadata staging
     HostBuf pinned2;
     std::vector<hipEvent_t> events;  // profiling event pool

@@ -99,7 +99,7 @@ __global__ __launch_bounds__(256) void k_seg_classify(const uint64_t* off, uint6
         rank = atomicAdd(&lcnt[8], 1u);
         rch = atomicAdd(&lch, nch);
         const unsigned long long need[NTOP] = {(unsigned long long)nch * B, F, B,
-                                               (unsigned long long)GUIDE_PER_SPLIT * F, nsamp};
+                                               (unsigned long long)guide_words(F), nsamp};
 #pragma unroll
         for (int q = 0; q < NTOP; ++q) rt[q] = atomicAdd(&ltop[q], need[q]);
     }
@@ -145,7 +145,7 @@ __global__ __launch_bounds__(256) void k_seg_classify(const uint64_t* off, uint6
     p.mode = 0;
     p.lshift = 0;
     p.lbase = 0;
-    p.pad = 0;
+    p.flags = 0;
     p.sample_base = btop[4] + rt[4];
     if (o >= c.nbig_cap) return;
     c.ps[o] = p;
@@ -212,7 +212,7 @@ static SegWs seg_layout(char* base, size_t nseg, size_t nkeys, const EngineParam
     w.sl = ar.take<TileSeg>(nbig_cap);
     w.samples = ar.take<K>(s_cap);
     w.trees = ar.take<K>(obkt);
-    w.guides = ar.take<uint32_t>(GUIDE_PER_SPLIT * obkt);
+    w.guides = ar.take<uint32_t>(17 * obkt + 4 * nbig_cap);
     w.hist = ar.take<uint32_t>(nchunk_cap * BMAX);
     w.bstart = ar.take<uint64_t>(2 * obkt);
     w.bsize = ar.take<uint64_t>(2 * obkt);
@@ -253,7 +253,7 @@ static SegWs seg_layout(char* base, size_t nseg, size_t nkeys, const EngineParam
     c.top_cap[0] = nchunk_cap * BMAX;
     c.top_cap[1] = obkt;
     c.top_cap[2] = 2 * obkt;
-    c.top_cap[3] = GUIDE_PER_SPLIT * obkt;
+    c.top_cap[3] = 17 * obkt + 4 * nbig_cap;  // guide_words(F) <= 17 F + 4
     c.top_cap[4] = s_cap;
     c.target_leaf = prm.target_leaf;
     c.oversample = prm.oversample;
@@ -289,21 +289,23 @@ static s512_status seg_enqueue(T* a, T* b, const uint64_t* off, size_t nseg, con
     k_build_tree<K><<<grid_of(nbig_cap, 2048), FMAX, 0, st>>>(w.ps, nbig, (K*)w.samples, SEG_SMAX, (K*)w.trees,
                                                               w.guides);
     S512_CHECK_LAUNCH();
-    k_count<T><<<grid_of(nchunk_cap, 8192), CNT_P, 0, st>>>(w.ps, w.cr, nchunk, a, b, (K*)w.trees, w.guides, w.hist,
-                                                           w.counters + CNT_SPECIAL, no_model);
+    launch_count<T>(prm.count_shape, grid_of(nchunk_cap, 8192), st, w.ps, w.cr, nchunk, a, b, (K*)w.trees, w.guides,
+                    w.hist, w.counters + CNT_SPECIAL, no_model, nullptr);
     S512_CHECK_LAUNCH();
-    k_colscan<<<grid_of(nbig_cap, 2048), BMAX, 0, st>>>(w.ps, nbig, w.hist, w.bstart, w.bsize);
+    k_colscan<<<grid_of((uint64_t)nbig_cap * CS_NSLICE, 8192), 256, 0, st>>>(w.ps, nbig, w.hist, w.bsize);
     S512_CHECK_LAUNCH();
-    k_scatter<T, false><<<grid_of(nchunk_cap, 4096), SCAT_P, 0, st>>>(w.ps, w.cr, a, b, nullptr, nullptr,
-                                                                      (K*)w.trees, w.guides, w.hist, w.bstart,
-                                                                      no_model, nchunk);
+    k_bucket_scan<<<grid_of(nbig_cap, 2048), BMAX, 0, st>>>(w.ps, nbig, w.bsize, w.bstart);
+    S512_CHECK_LAUNCH();
+    launch_scatter<T, false>(prm.scatter_shape, false, grid_of(nchunk_cap, 4096), st, w.ps, w.cr, a, b, nullptr,
+                             nullptr, (K*)w.trees, w.guides, w.hist, w.bstart, no_model, nchunk, nullptr);
     S512_CHECK_LAUNCH();
     k_plan<<<dim3(grid_of(nbig_cap, 2048), BMAX / 256), 256, 0, st>>>(w.ps, nbig, w.bstart, w.bsize, w.cls.pl);
     S512_CHECK_LAUNCH();
     k_copy_back<T, false><<<2048, 256, 0, st>>>(w.copy, w.counters + CNT_COPY, a, b, nullptr, nullptr);
     S512_CHECK_LAUNCH();
     for (uint32_t k = 0; k < w.cls.pl.nclass; ++k) {
-        launch_tile_fixed<T>((int)k, w.tile[k], w.counters + k, grid_of(w.cls.list_cap[k], 32768), a, b, st);
+        launch_tile_fixed<T>((int)k, prm.tile_plan, w.tile[k], w.counters + k, grid_of(w.cls.list_cap[k], 32768), a, b,
+                             st);
         S512_CHECK_LAUNCH();
     }
     return S512_OK;

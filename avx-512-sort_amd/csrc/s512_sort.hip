@@ -25,46 +25,90 @@ namespace s512 {
 static const uint32_t kCaps32[] = {256, 512, 1024, 2048, 4096, 8192, 16384};
 static const uint32_t kCaps64[] = {256, 512, 1024, 2048, 4096, 8192};
 
-template <class T>
-static void launch_tile_keys(int cls, const TileSeg* segs, const uint32_t* dcount, uint32_t count, T* a,
-                             const T* b, hipStream_t st) {
-    using K = typename KeyTraits<T>::K;
+// Tile shapes (threads P x keys per thread E) per class and plan.  Plan 0
+// keeps 8 keys per thread up to 2048-key tiles; plan 1 uses 16; plan 2 uses 32
+// (a 2048-key tile is one wave: every stage above the register window is one
+// wave-local LDS round trip, no workgroup barrier).  Every shape's LDS layout
+// is conflict-free (BlockBitonic::phys).
+constexpr int TILE_PLANS = 3;
+
+template <class T, class Pol>
+static void launch_tile_keys_pol(int cls, int plan, const TileSeg* segs, const uint32_t* dcount, uint32_t count, T* a,
+                                 const T* b, hipStream_t st) {
+    using K = typename Pol::key_t;
     if (count == 0) return;
     const uint32_t gw = (count + 3) / 4;
+#define S512_TILE(PP, EE) k_tile_sort<T, PP, EE, Pol><<<count, PP, 0, st>>>(segs, dcount, a, b)
     if constexpr (sizeof(K) == 4) {
-        switch (cls) {
-            case 0: k_wave_sort<T, 4><<<gw, 256, 0, st>>>(segs, dcount, a, b); break;
-            case 1: k_wave_sort<T, 8><<<gw, 256, 0, st>>>(segs, dcount, a, b); break;
-            case 2: k_tile_sort<T, 128, 8><<<count, 128, 0, st>>>(segs, dcount, a, b); break;
-            case 3: k_tile_sort<T, 256, 8><<<count, 256, 0, st>>>(segs, dcount, a, b); break;
-            case 4: k_tile_sort<T, 256, 16><<<count, 256, 0, st>>>(segs, dcount, a, b); break;
-            case 5: k_tile_sort<T, 512, 16><<<count, 512, 0, st>>>(segs, dcount, a, b); break;
-            default: k_tile_sort<T, 1024, 16><<<count, 1024, 0, st>>>(segs, dcount, a, b); break;
+        if (cls == 0) { k_wave_sort<T, 4, Pol><<<gw, 256, 0, st>>>(segs, dcount, a, b); return; }
+        if (cls == 1) { k_wave_sort<T, 8, Pol><<<gw, 256, 0, st>>>(segs, dcount, a, b); return; }
+        switch (plan * 8 + cls) {
+            case 2: S512_TILE(128, 8); break;
+            case 3: S512_TILE(256, 8); break;
+            case 4: S512_TILE(256, 16); break;
+            case 5: S512_TILE(512, 16); break;
+            case 8 + 2: S512_TILE(64, 16); break;
+            case 8 + 3: S512_TILE(128, 16); break;
+            case 8 + 4: S512_TILE(256, 16); break;
+            case 8 + 5: S512_TILE(512, 16); break;
+            case 16 + 2: S512_TILE(64, 16); break;
+            case 16 + 3: S512_TILE(64, 32); break;
+            case 16 + 4: S512_TILE(128, 32); break;
+            case 16 + 5: S512_TILE(256, 32); break;
+            case 16 + 6: S512_TILE(512, 32); break;
+            default: S512_TILE(1024, 16); break;  // class 6 of plans 0, 1
         }
     } else {
-        switch (cls) {
-            case 0: k_wave_sort<T, 4><<<gw, 256, 0, st>>>(segs, dcount, a, b); break;
-            case 1: k_tile_sort<T, 128, 4><<<count, 128, 0, st>>>(segs, dcount, a, b); break;
-            case 2: k_tile_sort<T, 128, 8><<<count, 128, 0, st>>>(segs, dcount, a, b); break;
-            case 3: k_tile_sort<T, 256, 8><<<count, 256, 0, st>>>(segs, dcount, a, b); break;
-            case 4: k_tile_sort<T, 256, 16><<<count, 256, 0, st>>>(segs, dcount, a, b); break;
-            default: k_tile_sort<T, 512, 16><<<count, 512, 0, st>>>(segs, dcount, a, b); break;
+        if (cls == 0) { k_wave_sort<T, 4, Pol><<<gw, 256, 0, st>>>(segs, dcount, a, b); return; }
+        switch (plan * 8 + cls) {
+            case 1: S512_TILE(128, 4); break;
+            case 2: S512_TILE(128, 8); break;
+            case 3: S512_TILE(256, 8); break;
+            case 4: S512_TILE(256, 16); break;
+            case 8 + 1: S512_TILE(64, 8); break;
+            case 8 + 2: S512_TILE(64, 16); break;
+            case 8 + 3: S512_TILE(128, 16); break;
+            case 8 + 4: S512_TILE(256, 16); break;
+            case 16 + 1: S512_TILE(64, 8); break;
+            case 16 + 2: S512_TILE(64, 16); break;
+            case 16 + 3: S512_TILE(64, 32); break;
+            case 16 + 4: S512_TILE(128, 32); break;
+            case 16 + 5: S512_TILE(256, 32); break;
+            default: S512_TILE(512, 16); break;  // class 5 of plans 0, 1
         }
     }
+#undef S512_TILE
 }
 
-static void launch_tile_kv(int cls, const TileSeg* segs, const uint32_t* dcount, uint32_t count, int32_t* ka,
-                           int32_t* va, const int32_t* kb, const int32_t* vb, hipStream_t st) {
+template <class T>
+static void launch_tile_keys(int cls, int plan, const TileSeg* segs, const uint32_t* dcount, uint32_t count, T* a,
+                             const T* b, hipStream_t st) {
+    launch_tile_keys_pol<T, DefaultOrder<T>>(cls, plan, segs, dcount, count, a, b, st);
+}
+
+static void launch_tile_kv(int cls, int plan, const TileSeg* segs, const uint32_t* dcount, uint32_t count,
+                           int32_t* ka, int32_t* va, const int32_t* kb, const int32_t* vb, hipStream_t st) {
     if (count == 0) return;
     const uint32_t gw = (count + 3) / 4;
-    switch (cls) {
-        case 0: k_wave_sort_kv<4><<<gw, 256, 0, st>>>(segs, dcount, ka, va, kb, vb); break;
-        case 1: k_tile_sort_kv<128, 4><<<count, 128, 0, st>>>(segs, dcount, ka, va, kb, vb); break;
-        case 2: k_tile_sort_kv<128, 8><<<count, 128, 0, st>>>(segs, dcount, ka, va, kb, vb); break;
-        case 3: k_tile_sort_kv<256, 8><<<count, 256, 0, st>>>(segs, dcount, ka, va, kb, vb); break;
-        case 4: k_tile_sort_kv<256, 16><<<count, 256, 0, st>>>(segs, dcount, ka, va, kb, vb); break;
-        default: k_tile_sort_kv<512, 16><<<count, 512, 0, st>>>(segs, dcount, ka, va, kb, vb); break;
+#define S512_TKV(PP, EE) k_tile_sort_kv<PP, EE><<<count, PP, 0, st>>>(segs, dcount, ka, va, kb, vb)
+    if (cls == 0) { k_wave_sort_kv<4><<<gw, 256, 0, st>>>(segs, dcount, ka, va, kb, vb); return; }
+    switch (plan * 8 + cls) {
+        case 1: S512_TKV(128, 4); break;
+        case 2: S512_TKV(128, 8); break;
+        case 3: S512_TKV(256, 8); break;
+        case 4: S512_TKV(256, 16); break;
+        case 8 + 1: S512_TKV(64, 8); break;
+        case 8 + 2: S512_TKV(64, 16); break;
+        case 8 + 3: S512_TKV(128, 16); break;
+        case 8 + 4: S512_TKV(256, 16); break;
+        case 16 + 1: S512_TKV(64, 8); break;
+        case 16 + 2: S512_TKV(64, 16); break;
+        case 16 + 3: S512_TKV(64, 32); break;
+        case 16 + 4: S512_TKV(128, 32); break;
+        case 16 + 5: S512_TKV(256, 32); break;
+        default: S512_TKV(512, 16); break;
     }
+#undef S512_TKV
 }
 
 template <class T, bool KV>
@@ -81,13 +125,13 @@ struct TileCfg {
 };
 
 template <class T, bool KV>
-static void launch_tile(int cls, const TileSeg* segs, const uint32_t* dcount, uint32_t count, T* a, int32_t* va,
-                        const T* b, const int32_t* vb, hipStream_t st, bool f64_fast = false) {
-    if constexpr (KV) launch_tile_kv(cls, segs, dcount, count, a, va, b, vb, st);
+static void launch_tile(int cls, int plan, const TileSeg* segs, const uint32_t* dcount, uint32_t count, T* a,
+                        int32_t* va, const T* b, const int32_t* vb, hipStream_t st, bool f64_fast = false) {
+    if constexpr (KV) launch_tile_kv(cls, plan, segs, dcount, count, a, va, b, vb, st);
     else if constexpr (std::is_same<T, double>::value) {
-        if (f64_fast) launch_tile_f64_fast(cls, segs, dcount, count, a, b, st);
-        else launch_tile_keys<T>(cls, segs, dcount, count, a, b, st);
-    } else launch_tile_keys<T>(cls, segs, dcount, count, a, b, st);
+        if (f64_fast) launch_tile_f64_fast(cls, plan, segs, dcount, count, a, b, st);
+        else launch_tile_keys<T>(cls, plan, segs, dcount, count, a, b, st);
+    } else launch_tile_keys<T>(cls, plan, segs, dcount, count, a, b, st);
 }
 
 static inline int ceil_log2(uint64_t x) {
@@ -103,15 +147,15 @@ const uint32_t* tile_caps(int* ncls) {
     return TileCfg<T, false>::caps();
 }
 template <class T>
-void launch_tile_fixed(int cls, const TileSeg* segs, const uint32_t* dcount, uint32_t grid_items, T* a, const T* b,
-                       hipStream_t st) {
-    launch_tile_keys<T>(cls, segs, dcount, grid_items, a, b, st);
+void launch_tile_fixed(int cls, int plan, const TileSeg* segs, const uint32_t* dcount, uint32_t grid_items, T* a,
+                       const T* b, hipStream_t st) {
+    launch_tile_keys<T>(cls, plan, segs, dcount, grid_items, a, b, st);
 }
 template const uint32_t* tile_caps<int32_t>(int*);
 template const uint32_t* tile_caps<double>(int*);
-template void launch_tile_fixed<int32_t>(int, const TileSeg*, const uint32_t*, uint32_t, int32_t*, const int32_t*,
-                                         hipStream_t);
-template void launch_tile_fixed<double>(int, const TileSeg*, const uint32_t*, uint32_t, double*, const double*,
+template void launch_tile_fixed<int32_t>(int, int, const TileSeg*, const uint32_t*, uint32_t, int32_t*,
+                                         const int32_t*, hipStream_t);
+template void launch_tile_fixed<double>(int, int, const TileSeg*, const uint32_t*, uint32_t, double*, const double*,
                                         hipStream_t);
 
 // ---------------------------------------------------------------- engine
@@ -137,6 +181,11 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
     uint64_t tiles = 0;
     uint64_t keys_ph[S512_NPHASE] = {0};
     bool f64_fast = false;
+    uint16_t* d_ids = nullptr;
+    if (prm.ids) {
+        if ((rc = ds.ids.reserve(n * sizeof(uint16_t), st))) return rc;
+        d_ids = (uint16_t*)ds.ids.ptr;
+    }
     while (!segs.empty()) {
         if (++level > prm.max_levels) {
             set_error("partition did not converge within %d levels", prm.max_levels);
@@ -187,10 +236,10 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
             p.kmin = 0;
             p.shift = 0;
             if (p.mode) p.nsample = 0;
-            p.pad = 0;
+            p.flags = 0;
             p.sample_base = (uint64_t)i * smax;
             sumF += F;
-            sumG += (uint64_t)GUIDE_PER_SPLIT * F;
+            sumG += guide_words(F);
             histN += (uint64_t)p.nchunks * B;
             sumB += B;
             maxB = std::max<uint64_t>(maxB, B);
@@ -284,12 +333,14 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
         pf.end();
         }
         pf.begin(S512_PHASE_COUNT);
-        k_count<T><<<tc, CNT_P, 0, st>>>(d_ps, d_cr, d_cnt + 13, a, b, d_tree, d_guide, d_hist, d_cnt + 11,
-                                         model.table);
+        launch_count<T>(prm.count_shape, tc, st, d_ps, d_cr, d_cnt + 13, a, b, d_tree, d_guide, d_hist, d_cnt + 11,
+                        model.table, d_ids);
         S512_CHECK_LAUNCH();
         pf.end();
         pf.begin(S512_PHASE_SCAN);
-        k_colscan<<<m, BMAX, 0, st>>>(d_ps, d_cnt + 12, d_hist, d_bstart, d_bsize);
+        k_colscan<<<m * CS_NSLICE, 256, 0, st>>>(d_ps, d_cnt + 12, d_hist, d_bsize);
+        S512_CHECK_LAUNCH();
+        k_bucket_scan<<<m, BMAX, 0, st>>>(d_ps, d_cnt + 12, d_bsize, d_bstart);
         S512_CHECK_LAUNCH();
         pf.end();
         if (any_learned && level == 1 && !learned_checked) {
@@ -315,8 +366,8 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
             if (stats) stats->learned_levels = lf2 ? 2 : 1;
         }
         pf.begin(S512_PHASE_SCATTER);
-        k_scatter<T, KV><<<tc, SCAT_P, 0, st>>>(d_ps, d_cr, a, b, va, vb, d_tree, d_guide, d_hist, d_bstart,
-                                                model.table, d_cnt + 13);
+        launch_scatter<T, KV>(prm.scatter_shape, d_ids != nullptr, tc, st, d_ps, d_cr, a, b, va, vb, d_tree,
+                              d_guide, d_hist, d_bstart, model.table, d_cnt + 13, d_ids);
         S512_CHECK_LAUNCH();
         pf.end();
         pf.begin(S512_PHASE_PLAN);
@@ -349,7 +400,7 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
         for (int c = 0; c < ncls; ++c) {
             if (!counts[c]) continue;
             pf.begin(S512_PHASE_TILE);
-            launch_tile<T, KV>(c, pl.tile[c], d_cnt + c, counts[c], a, va, b, vb, st, f64_fast);
+            launch_tile<T, KV>(c, prm.tile_plan, pl.tile[c], d_cnt + c, counts[c], a, va, b, vb, st, f64_fast);
             S512_CHECK_LAUNCH();
             pf.end();
             tiles += counts[c];
@@ -423,7 +474,7 @@ s512_status sort_engine_locked(T* a, int32_t* va, size_t n, hipStream_t st, cons
         *hcnt = 1;
         S512_HIP(hipMemcpyAsync(ds.arena.ptr, ds.pinned.ptr, sc.off, hipMemcpyHostToDevice, st));
         pf.begin(S512_PHASE_TILE);
-        launch_tile<T, KV>(TC::cls_of(n), dseg, dcnt, 1, a, va, a, va, st);
+        launch_tile<T, KV>(TC::cls_of(n), prm.tile_plan, dseg, dcnt, 1, a, va, a, va, st);
         S512_CHECK_LAUNCH();
         pf.end();
         S512_HIP(hipStreamSynchronize(st));  // pinned staging is reused by the next call
@@ -492,6 +543,10 @@ EngineParams default_params(bool wide) {
     p.train_steps = 100;
     p.learn_rate = 2e-3f;
     p.imbalance = 4.0f;
+    p.scatter_shape = 0;
+    p.ids = true;
+    p.count_shape = 2;  // 256 threads x 8 keys, 8 classified together (fastest measured)
+    p.tile_plan = 2;    // 32 keys per thread above 1024-key tiles (fastest measured)
     return p;
 }
 
@@ -506,6 +561,13 @@ EngineParams params_from(const s512_options* opt, bool wide) {
     if (opt->sample_permille) p.sample_permille = opt->sample_permille;
     if (opt->train_steps) p.train_steps = opt->train_steps;
     if (opt->imbalance_pct) p.imbalance = (float)opt->imbalance_pct / 100.0f;
+    if (opt->scatter_variant) {
+        const uint32_t v = opt->scatter_variant - 1;
+        p.scatter_shape = (int)(v % SCATTER_SHAPES);
+        p.ids = v < SCATTER_SHAPES;
+    }
+    if (opt->count_variant) p.count_shape = (int)opt->count_variant - 1;
+    if (opt->tile_variant) p.tile_plan = (int)opt->tile_variant - 1;
     return p;
 }
 
@@ -554,6 +616,18 @@ static s512_status check_opts(const s512_options* opt) {
     }
     if (opt && (opt->sample_permille > 1000 || opt->train_steps > 100000)) {
         set_error("sample_permille (<= 1000) or train_steps (<= 100000) out of range");
+        return S512_ERR_INVALID;
+    }
+    if (opt && opt->tile_variant > TILE_PLANS) {
+        set_error("tile_variant %u out of range (max %d)", opt->tile_variant, TILE_PLANS);
+        return S512_ERR_INVALID;
+    }
+    if (opt && opt->count_variant > COUNT_SHAPES) {
+        set_error("count_variant %u out of range (max %d)", opt->count_variant, COUNT_SHAPES);
+        return S512_ERR_INVALID;
+    }
+    if (opt && opt->scatter_variant > 2 * SCATTER_SHAPES) {
+        set_error("scatter_variant %u out of range (max %d)", opt->scatter_variant, 2 * SCATTER_SHAPES);
         return S512_ERR_INVALID;
     }
     if (opt && opt->target_leaf && opt->target_leaf < 64) {

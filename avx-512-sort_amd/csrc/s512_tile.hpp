@@ -169,7 +169,11 @@ struct BlockBitonic {
     // when 16 (32) are.  Found by exhaustive search over XOR taps; for the
     // configurations below every access is conflict-free in both models.
     __device__ static __forceinline__ uint32_t phys(uint32_t i) {
-        if constexpr (sizeof(K) == 8 && E == 16) {
+        if constexpr (sizeof(K) == 8 && E == 32) {
+            return i ^ ((i >> 5) & 31u);
+        } else if constexpr (sizeof(K) == 4 && E == 32) {
+            return i ^ ((i >> 5) & 63u);
+        } else if constexpr (sizeof(K) == 8 && E == 16) {
             return i ^ ((i >> 4) & 31u);
         } else if constexpr (sizeof(K) == 8 && E == 8) {
             return i ^ ((i >> 4) & 3u) ^ (((i >> 6) & 1u) * 12u) ^ (((i >> 7) & 1u) << 4);

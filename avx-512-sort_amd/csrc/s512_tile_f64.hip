@@ -13,18 +13,29 @@
 
 namespace s512 {
 
-void launch_tile_f64_fast(int cls, const TileSeg* segs, const uint32_t* dcount, uint32_t count, double* a,
-                          const double* b, hipStream_t st) {
+void launch_tile_f64_fast(int cls, int plan, const TileSeg* segs, const uint32_t* dcount, uint32_t count,
+                          double* a, const double* b, hipStream_t st) {
     if (count == 0) return;
     const uint32_t gw = (count + 3) / 4;
-    switch (cls) {
-        case 0: k_wave_sort<double, 4, F64Order><<<gw, 256, 0, st>>>(segs, dcount, a, b); break;
-        case 1: k_tile_sort<double, 128, 4, F64Order><<<count, 128, 0, st>>>(segs, dcount, a, b); break;
-        case 2: k_tile_sort<double, 128, 8, F64Order><<<count, 128, 0, st>>>(segs, dcount, a, b); break;
-        case 3: k_tile_sort<double, 256, 8, F64Order><<<count, 256, 0, st>>>(segs, dcount, a, b); break;
-        case 4: k_tile_sort<double, 256, 16, F64Order><<<count, 256, 0, st>>>(segs, dcount, a, b); break;
-        default: k_tile_sort<double, 512, 16, F64Order><<<count, 512, 0, st>>>(segs, dcount, a, b); break;
+#define S512_TF(PP, EE) k_tile_sort<double, PP, EE, F64Order><<<count, PP, 0, st>>>(segs, dcount, a, b)
+    if (cls == 0) { k_wave_sort<double, 4, F64Order><<<gw, 256, 0, st>>>(segs, dcount, a, b); return; }
+    switch (plan * 8 + cls) {
+        case 1: S512_TF(128, 4); break;
+        case 2: S512_TF(128, 8); break;
+        case 3: S512_TF(256, 8); break;
+        case 4: S512_TF(256, 16); break;
+        case 8 + 1: S512_TF(64, 8); break;
+        case 8 + 2: S512_TF(64, 16); break;
+        case 8 + 3: S512_TF(128, 16); break;
+        case 8 + 4: S512_TF(256, 16); break;
+        case 16 + 1: S512_TF(64, 8); break;
+        case 16 + 2: S512_TF(64, 16); break;
+        case 16 + 3: S512_TF(64, 32); break;
+        case 16 + 4: S512_TF(128, 32); break;
+        case 16 + 5: S512_TF(256, 32); break;
+        default: S512_TF(512, 16); break;
     }
+#undef S512_TF
 }
 
 }  // namespace s512

@@ -5,6 +5,6 @@
 
 namespace s512 {
 // Same classes as the 64-bit integer tile kernels (caps 256 .. 8192).
-void launch_tile_f64_fast(int cls, const TileSeg* segs, const uint32_t* dcount, uint32_t count, double* a,
-                          const double* b, hipStream_t st);
+void launch_tile_f64_fast(int cls, int plan, const TileSeg* segs, const uint32_t* dcount, uint32_t count,
+                          double* a, const double* b, hipStream_t st);
 }  // namespace s512

@@ -99,7 +99,9 @@ class Options(ctypes.Structure):
     _fields_ = [("target_leaf", ctypes.c_uint32), ("oversample", ctypes.c_uint32),
                 ("max_fanout_log2", ctypes.c_uint32), ("profile", ctypes.c_uint32), ("seed", ctypes.c_uint64),
                 ("classifier", ctypes.c_uint32), ("sample_permille", ctypes.c_uint32),
-                ("train_steps", ctypes.c_uint32), ("imbalance_pct", ctypes.c_uint32)]
+                ("train_steps", ctypes.c_uint32), ("imbalance_pct", ctypes.c_uint32),
+                ("scatter_variant", ctypes.c_uint32), ("count_variant", ctypes.c_uint32),
+                ("tile_variant", ctypes.c_uint32)]
 
 
 SPLITTERS, LEARNED = 0, 1

@@ -134,6 +134,17 @@ typedef struct s512_options {
     uint32_t train_steps;   /* learned: Adam steps (0: 100)                 */
     uint32_t imbalance_pct; /* learned: fall back to splitters when a first-level bucket
                                exceeds this percentage of the mean (0: 400) */
+    uint32_t scatter_variant; /* 0: default (1); else 1 + shape + 3 * reclassify, where shape
+                                 0..2 = scatter workgroup 512x16, 256x16, 1024x8 (threads x
+                                 keys per thread) and reclassify = 1 classifies every key in
+                                 the scatter again instead of reading the bucket ids that
+                                 the count pass stored (2 bytes per key)              */
+    uint32_t count_variant;   /* 0: default (shape 2); else 1 + count shape 0..4 = (threads,
+                                 keys per thread per batch, keys classified together):
+                                 256x16x8, 256x16x4, 256x8x8, 512x8x8, 256x8x4          */
+    uint32_t tile_variant;    /* 0: default (plan 2); else 1 + tile plan 0..2 = keys per
+                                 thread of the bitonic tile sorts: 8, 16 or 32 (up to 16
+                                 on 64-bit tiles of plan 0)                             */
 } s512_options;
 
 typedef struct s512_stats {

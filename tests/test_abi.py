@@ -53,6 +53,11 @@ def test_host_validation_without_gpu(s512):
     opt = s512.Options()
     opt.oversample = 100000
     assert lib.s512_sort_i32_dev_ex(ctypes.c_void_p(16), 100, ctypes.byref(opt), None, None) == 1
+    for field, bad in (("scatter_variant", 7), ("count_variant", 6), ("tile_variant", 4)):
+        opt = s512.Options()
+        setattr(opt, field, bad)
+        assert lib.s512_sort_f64_dev_ex(ctypes.c_void_p(16), 100, ctypes.byref(opt), None, None) == 1
+        assert field.split("_")[0] in lib.s512_last_error().decode()
     # synthetic generator: unknown distribution
     assert lib.s512x_fill_i32(ctypes.c_void_p(16), 10, 1, 0, 0, None) == 1
 

@@ -1,0 +1,97 @@
+"""GPU parity of every kernel shape the tuning options can select.
+
+The count, scatter and tile-sort kernels come in several shapes (workgroup
+threads, keys per thread, where the bucket of a key comes from); the
+autotuner picks among them per dtype and size class.  The output must not
+depend on the choice: each shape is checked against the oracle's closed form
+(ascending order is unique, sort512.hpp:5960) on inputs that reach every tile
+class and both partition levels, plus a skewed and a duplicate-heavy input
+that switch on the classifier's sub-tables and equality buckets.
+"""
+import numpy as np
+import pytest
+
+from gpu_util import assert_keys_equal, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+N = 3_000_001
+
+
+def _input(orc, dtype, kind):
+    if dtype == np.int32:
+        a = orc.gen_i32_full(N, 11)
+        if kind == "distinct":
+            a = (a % 1000).astype(np.int32)
+        elif kind == "sorted":
+            a = np.sort(a)
+        return a
+    return orc.gen_f64(kind, N, 11)
+
+
+def _run(s512, dev, a, **opts):
+    t = to_dev(a, dev)
+    o = s512.Options()
+    for k, v in opts.items():
+        setattr(o, k, v)
+    st = s512.sort_ex(t, o)
+    return to_host(t), st
+
+
+@pytest.mark.parametrize("kind", ["normal", "distinct", "sorted"])
+@pytest.mark.parametrize("dtype", [np.int32, np.float64])
+@pytest.mark.parametrize("variant", range(1, 7))
+def test_scatter_variants(s512, orc, dev, dtype, kind, variant):
+    a = _input(orc, dtype, kind)
+    got, st = _run(s512, dev, a, scatter_variant=variant)
+    assert_keys_equal(got, orc.closed_sort(a.copy()), f"scatter_variant={variant} {kind}")
+    assert st.levels >= 1
+
+
+@pytest.mark.parametrize("kind", ["normal", "distinct"])
+@pytest.mark.parametrize("dtype", [np.int32, np.float64])
+@pytest.mark.parametrize("variant", range(1, 6))
+def test_count_variants(s512, orc, dev, dtype, kind, variant):
+    a = _input(orc, dtype, kind)
+    got, _ = _run(s512, dev, a, count_variant=variant)
+    assert_keys_equal(got, orc.closed_sort(a.copy()), f"count_variant={variant} {kind}")
+
+
+@pytest.mark.parametrize("n", [200, 700, 1500, 3000, 6000, 12000, 100_000, N])
+@pytest.mark.parametrize("dtype", [np.int32, np.float64])
+@pytest.mark.parametrize("variant", range(1, 4))
+def test_tile_variants(s512, orc, dev, dtype, n, variant):
+    a = _input(orc, dtype, "normal")[:n].copy()
+    got, _ = _run(s512, dev, a, tile_variant=variant)
+    assert_keys_equal(got, orc.closed_sort(a.copy()), f"tile_variant={variant} n={n}")
+
+
+@pytest.mark.parametrize("variant", range(1, 4))
+def test_tile_variants_kv(s512, orc, dev, variant):
+    k = (orc.gen_i32_full(N, 5) & 0xFFFFF).astype(np.int32)
+    v = np.arange(N, dtype=np.int32)
+    tk, tv = to_dev(k, dev), to_dev(v, dev)
+    o = s512.Options()
+    o.tile_variant = variant
+    s512.kv_sort_ex(tk, tv, o)
+    gk, gv = to_host(tk), to_host(tv)
+    assert np.array_equal(gk, orc.closed_sort(k.copy()))
+    assert np.array_equal(orc.pair_multiset(gk, gv), orc.pair_multiset(k, v))
+
+
+def test_fp64_minmax_tiles_with_variants(s512, orc, dev):
+    """fp64 inputs without NaN/-0.0 take the v_min/v_max tile kernels."""
+    a = orc.gen_f64("uniform", N, 3)
+    for variant in range(1, 4):
+        got, _ = _run(s512, dev, a, tile_variant=variant)
+        assert np.array_equal(got.view(np.uint64), np.sort(a).view(np.uint64)), variant
+
+
+def test_variant_out_of_range(s512, dev):
+    import torch
+    t = torch.zeros(10, dtype=torch.float64, device=dev)
+    for field, bad in (("scatter_variant", 7), ("count_variant", 6), ("tile_variant", 4)):
+        o = s512.Options()
+        setattr(o, field, bad)
+        with pytest.raises(s512.S512Error):
+            s512.sort_ex(t, o)
