@@ -29,7 +29,7 @@ constexpr int FMAX_LOG = 9;                 // at most 512-way fan-out per level
 constexpr int FMAX = 1 << FMAX_LOG;
 constexpr int BMAX = 2 * FMAX;              // ordinary + equality buckets
 constexpr int SCAT_ROUND = 8192;            // chunks are multiples of this many keys
-constexpr int SCATTER_SHAPES = 3;           // (threads, keys per thread): 512x16, 256x16, 1024x8
+constexpr int SCATTER_SHAPES = 4;           // 512x16, 256x16, 1024x8, 512x16 write-combining
 constexpr int COUNT_SHAPES = 5;            // (threads, keys per batch, classified together), see launch_count
 constexpr uint32_t COPY_PIECE = 1u << 16;   // keys per copy work item
 constexpr int CLS_NB = 8;                   // keys classified together (overlapped LDS reads)
@@ -840,7 +840,14 @@ constexpr int CS_BATCH = 8;
 //   stage keys grouped by bucket in LDS                      | barrier 4
 //   write the staged keys: consecutive threads, consecutive addresses
 // The next round's keys are loaded into registers while this round runs.
-template <class T, bool KV, bool IDS, int P, int E>
+// Write combining (WC): a bucket's run usually ends inside a 128-byte line
+// whose rest the next round writes; the keys of that unfinished line wait in
+// an LDS list and are written during the next round's staging phase, a few
+// microseconds before the rest of their line, instead of a whole round
+// earlier (the L2 then merges the two halves rather than evicting half a line).
+constexpr int WC_CAP = 4096;  // deferred tail-line keys per round (write combining)
+
+template <class T, bool KV, bool IDS, int P, int E, bool WC>
 struct ScatterLds {
     using K = typename KeyTraits<T>::K;
     static constexpr int ROUND = P * E;
@@ -856,16 +863,21 @@ struct ScatterLds {
     T stg[ROUND];
     uint16_t stb[ROUND];
     int32_t stv[KV ? ROUND : 1];
+    T dkey[WC ? WC_CAP : 1];              // deferred keys (write combining)
+    int32_t dval[WC && KV ? WC_CAP : 1];
+    uint32_t dpos[WC ? WC_CAP : 1];
+    uint32_t dcount;
 };
 
-template <class T, bool KV, bool IDS, int P, int E>
+template <class T, bool KV, bool IDS, int P, int E, bool WC>
 __global__ __launch_bounds__(P) void k_scatter(const PartSeg* ps, const ChunkRef* cr, T* a, T* b, int32_t* va,
                                                int32_t* vb, const typename KeyTraits<T>::K* trees,
                                                const uint32_t* guides, const uint32_t* hist, const uint64_t* bstart,
                                                LearnedTable lt, const uint32_t* nchunk, const uint16_t* ids) {
     using Tr = KeyTraits<T>;
     using K = typename Tr::K;
-    using L = ScatterLds<T, KV, IDS, P, E>;
+    using L = ScatterLds<T, KV, IDS, P, E, WC>;
+    constexpr uint64_t LINE = 128 / sizeof(T);  // keys per 128-byte line
     constexpr int NW = P / 64;
     constexpr int ROUND = L::ROUND;
     constexpr int BPT = BMAX / P;  // buckets per thread in the scan
@@ -884,11 +896,18 @@ __global__ __launch_bounds__(P) void k_scatter(const PartSeg* ps, const ChunkRef
         sm.cursor[j] = j < B ? (uint32_t)bstart[s.bkt_base + j] + h[j] : 0u;
         sm.rh[j] = 0;
     }
+    if (WC && t == 0) sm.dcount = 0;
     const T* src = (s.buf ? b : a) + s.begin;
     T* dst = (s.buf ? a : b) + s.begin;
     const int32_t* vsrc = KV ? ((s.buf ? vb : va) + s.begin) : nullptr;
     int32_t* vdst = KV ? ((s.buf ? va : vb) + s.begin) : nullptr;
     const uint16_t* isrc = IDS ? ids + s.begin : nullptr;
+    auto flush_deferred = [&](uint32_t nd) {
+        for (uint32_t k = t; k < nd; k += P) {
+            dst[sm.dpos[k]] = sm.dkey[k];
+            if constexpr (KV) vdst[sm.dpos[k]] = sm.dval[k];
+        }
+    };
     const uint64_t lo = (uint64_t)c.idx * s.chunk;
     const uint64_t hi = lo + s.chunk < s.size ? lo + s.chunk : s.size;
 
@@ -958,6 +977,7 @@ __global__ __launch_bounds__(P) void k_scatter(const PartSeg* ps, const ChunkRef
         uint32_t woff = 0;
 #pragma unroll
         for (int w = 0; w < NW; ++w) woff += (w < (int)wid) ? sm.wtot[w] : 0u;
+        const uint32_t nd = WC ? sm.dcount : 0u;  // deferred by the previous round
         uint32_t p = woff + incl - v;
 #pragma unroll
         for (int k = 0; k < BPT; ++k) {
@@ -969,6 +989,10 @@ __global__ __launch_bounds__(P) void k_scatter(const PartSeg* ps, const ChunkRef
             p += hv[k];
         }
         __syncthreads();  // 3
+        if constexpr (WC) {
+            flush_deferred(nd < WC_CAP ? nd : WC_CAP);
+            if (t == 0) sm.dcount = 0;  // every thread read it before barrier 3
+        }
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const uint32_t i = e * P + t;
@@ -989,8 +1013,24 @@ __global__ __launch_bounds__(P) void k_scatter(const PartSeg* ps, const ChunkRef
             int32_t yv = 0;
             if constexpr (KV) yv = sm.stv[i];
             if (i < cnt) {
-                dst[d] = y;
-                if constexpr (KV) vdst[d] = yv;
+                bool direct = true;
+                if constexpr (WC) {
+                    // keys in the unfinished last line of this bucket's run wait
+                    const uint64_t end = s.begin + sm.cursor[bkt];  // absolute end after this round
+                    if ((end & (LINE - 1)) && s.begin + d >= (end & ~(LINE - 1))) {
+                        const uint32_t k = atomicAdd(&sm.dcount, 1u);
+                        if (k < WC_CAP) {
+                            sm.dkey[k] = y;
+                            sm.dpos[k] = (uint32_t)d;
+                            if constexpr (KV) sm.dval[k] = yv;
+                            direct = false;
+                        }
+                    }
+                }
+                if (direct) {
+                    dst[d] = y;
+                    if constexpr (KV) vdst[d] = yv;
+                }
             }
         }
         // the next round's atomics touch rh (cleared before barrier 3) and the
@@ -1007,6 +1047,11 @@ __global__ __launch_bounds__(P) void k_scatter(const PartSeg* ps, const ChunkRef
         do_round(base, Bb, A);
         base += ROUND;
     }
+    if constexpr (WC) {
+        __syncthreads();
+        const uint32_t nd = sm.dcount;
+        flush_deferred(nd < WC_CAP ? nd : WC_CAP);
+    }
     __syncthreads();  // LDS is reloaded for the next chunk
     }
 }
@@ -1019,20 +1064,22 @@ inline void launch_scatter(int shape, bool use_ids, uint32_t grid, hipStream_t s
                            const typename KeyTraits<T>::K* trees, const uint32_t* guides, const uint32_t* hist,
                            const uint64_t* bstart, const LearnedTable& lt, const uint32_t* nchunk,
                            const uint16_t* ids) {
-#define S512_SCATTER(I, PP, EE)                                                                               \
-    k_scatter<T, KV, I, PP, EE><<<grid, PP, 0, st>>>(ps, cr, a, b, va, vb, trees, guides, hist, bstart, lt, nchunk, \
-                                                     ids)
+#define S512_SCATTER(I, PP, EE, W)                                                                              \
+    k_scatter<T, KV, I, PP, EE, W><<<grid, PP, 0, st>>>(ps, cr, a, b, va, vb, trees, guides, hist, bstart, lt, nchunk, \
+                                                        ids)
     if (use_ids) {
         switch (shape) {
-            case 1: S512_SCATTER(true, 256, 16); break;
-            case 2: S512_SCATTER(true, 1024, 8); break;
-            default: S512_SCATTER(true, 512, 16); break;
+            case 1: S512_SCATTER(true, 256, 16, false); break;
+            case 2: S512_SCATTER(true, 1024, 8, false); break;
+            case 3: S512_SCATTER(true, 512, 16, true); break;
+            default: S512_SCATTER(true, 512, 16, false); break;
         }
     } else {
+        // write combining needs the LDS the classifier tables take: shape 3 runs as shape 0
         switch (shape) {
-            case 1: S512_SCATTER(false, 256, 16); break;
-            case 2: S512_SCATTER(false, 1024, 8); break;
-            default: S512_SCATTER(false, 512, 16); break;
+            case 1: S512_SCATTER(false, 256, 16, false); break;
+            case 2: S512_SCATTER(false, 1024, 8, false); break;
+            default: S512_SCATTER(false, 512, 16, false); break;
         }
     }
 #undef S512_SCATTER

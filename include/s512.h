@@ -124,7 +124,7 @@ enum {
 };
 
 typedef struct s512_options {
-    uint32_t target_leaf;   /* mean leaf size the fan-out aims for (0: 2048 for 64-bit keys, 4096 for 32-bit) */
+    uint32_t target_leaf;   /* mean leaf size the fan-out aims for (0: 1024)     */
     uint32_t oversample;    /* samples per splitter (0: 16)                  */
     uint32_t max_fanout_log2; /* cap on log2 of the per-level fan-out, 1..9 (0: 9) */
     uint32_t profile;       /* nonzero: time every kernel with HIP events    */
@@ -134,11 +134,13 @@ typedef struct s512_options {
     uint32_t train_steps;   /* learned: Adam steps (0: 100)                 */
     uint32_t imbalance_pct; /* learned: fall back to splitters when a first-level bucket
                                exceeds this percentage of the mean (0: 400) */
-    uint32_t scatter_variant; /* 0: default (1); else 1 + shape + 3 * reclassify, where shape
-                                 0..2 = scatter workgroup 512x16, 256x16, 1024x8 (threads x
-                                 keys per thread) and reclassify = 1 classifies every key in
-                                 the scatter again instead of reading the bucket ids that
-                                 the count pass stored (2 bytes per key)              */
+    uint32_t scatter_variant; /* 0: default (1); else 1 + shape + 4 * reclassify, where shape
+                                 0..3 = scatter workgroup 512x16, 256x16, 1024x8 (threads x
+                                 keys per thread), 512x16 with write combining (only with
+                                 stored ids; reclassifying runs it as shape 0), and
+                                 reclassify = 1 classifies every key in the scatter again
+                                 instead of reading the bucket ids that the count pass
+                                 stored (2 bytes per key)                             */
     uint32_t count_variant;   /* 0: default (shape 2); else 1 + count shape 0..4 = (threads,
                                  keys per thread per batch, keys classified together):
                                  256x16x8, 256x16x4, 256x8x8, 512x8x8, 256x8x4          */

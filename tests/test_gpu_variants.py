@@ -40,7 +40,7 @@ def _run(s512, dev, a, **opts):
 
 @pytest.mark.parametrize("kind", ["normal", "distinct", "sorted"])
 @pytest.mark.parametrize("dtype", [np.int32, np.float64])
-@pytest.mark.parametrize("variant", range(1, 7))
+@pytest.mark.parametrize("variant", range(1, 9))
 def test_scatter_variants(s512, orc, dev, dtype, kind, variant):
     a = _input(orc, dtype, kind)
     got, st = _run(s512, dev, a, scatter_variant=variant)
@@ -90,7 +90,7 @@ def test_fp64_minmax_tiles_with_variants(s512, orc, dev):
 def test_variant_out_of_range(s512, dev):
     import torch
     t = torch.zeros(10, dtype=torch.float64, device=dev)
-    for field, bad in (("scatter_variant", 7), ("count_variant", 6), ("tile_variant", 4)):
+    for field, bad in (("scatter_variant", 9), ("count_variant", 6), ("tile_variant", 4)):
         o = s512.Options()
         setattr(o, field, bad)
         with pytest.raises(s512.S512Error):
