@@ -8,23 +8,39 @@ distinct values, sorted ascending, sorted descending.  One step sorts each of
 the five arrays once (5 x 2^28 keys); the unsorted input is restored from a
 pristine copy before every sort, outside the timed events.
 
-Output: one JSON line.  `value` = keys sorted / device time of the sorts
-(HIP events on the stream the library runs on).  `roofline` describes the
-kernel with the largest share of device time, timed per launch with HIP
-events inside the library (s512_options.profile).  `cpu_baseline` times the
-reference's SortOmpPartition (compiled from the reference headers into
-oracle/_ref when they were available at build time) on a bounded sample on
-this host's cores; if that library or AVX-512 is absent, the oracle's C
-restatement (one core) is timed instead.
+Output: one JSON line.
+  value         keys sorted / device time of the sorts (HIP events on the
+                stream the library runs on)
+  roofline      the kernel phase with the largest share of device time, timed
+                per launch with HIP events inside the library
+                (s512_options.profile); algorithmic bytes = one read and one
+                write of every key (16 B per fp64 key) per scatter / tile /
+                copy launch, one read (8 B) per count launch
+  traffic       HBM bytes per launch of that kernel, from two rocprofv3 --pmc
+                child runs of this script on the uniform input (FETCH_SIZE,
+                then WRITE_SIZE), each counter scaled by what it reads for the
+                run's own 2 GiB restore copy (a copy kernel of known size)
+  learned       the learned partitioner against sampled splitters on the
+                skewed mixture and on uniform keys (same input, both exact)
+  cpu_baseline  the reference's SortOmpPartition, compiled from the reference
+                headers into oracle/_ref when they were available at build
+                time, on a bounded sample on this host's cores; if that
+                library or AVX-512 is absent, the oracle's C restatement (one
+                core) is timed instead
 
 Every sorted array is verified at full size by size-independent properties:
 non-decreasing order and an order-independent multiset hash equal to the
 input's.
 """
 import argparse
+import csv
+import glob
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -32,6 +48,7 @@ sys.path.insert(0, os.path.join(ROOT, "avx-512-sort_amd"))
 
 HBM_PEAK_GBS = 8000.0
 DISTS = ["uniform", "normal", "distinct", "sorted", "reversed"]
+KERNEL_OF_PHASE = {"count": "k_count", "scatter": "k_scatter", "tile": "k_tile_sort", "copy": "k_copy_back"}
 
 
 def parse():
@@ -42,6 +59,8 @@ def parse():
     ap.add_argument("--n", type=int, default=268_435_456)
     ap.add_argument("--dists", default=",".join(DISTS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 --pmc child runs")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-splitters comparison")
     ap.add_argument("--cpu-sample", type=int, default=1 << 25)
     return ap.parse_args()
 
@@ -100,12 +119,97 @@ def cpu_baseline(sample_n):
             "sample": f"{what}, {sample_n} fp64 keys uniform [0,1) (SplitMix64 seed 0), best of {runs}"}
 
 
+# ------------------------------------------------------------------ traffic
+def _pmc_run(counter, n, tmp):
+    """One rocprofv3 --pmc child run of this script; returns rows of its CSV."""
+    out = os.path.join(tmp, counter)
+    cmd = ["rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "p", "--",
+           sys.executable, os.path.abspath(__file__), "--steps", "1", "--warmup", "0", "--n", str(n),
+           "--dists", "uniform", "--no-cpu-baseline", "--no-traffic", "--no-learned"]
+    subprocess.run(cmd, check=True, timeout=240, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise RuntimeError(f"rocprofv3 wrote no counter file for {counter}")
+    with open(files[0]) as f:
+        return list(csv.DictReader(f))
+
+
+def measure_traffic(n, kernel):
+    """HBM bytes per dispatch of `kernel` (mean over its dispatches), each
+    counter scaled by its reading for the 2 GiB restore copy of the run."""
+    if shutil.which("rocprofv3") is None:
+        return None, "rocprofv3 not found"
+    tmp = tempfile.mkdtemp(prefix="s512_pmc_")
+    try:
+        copy_bytes = n * 8
+        res = {}
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            rows = _pmc_run(counter, n, tmp)
+            copies = [float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"]]
+            if not copies:
+                raise RuntimeError("no copy kernel in the counter run")
+            scale = copy_bytes / (max(copies) * 1024.0)  # the restore copy is the largest
+            vals = [float(r["Counter_Value"]) * 1024.0 * scale for r in rows if kernel in r["Kernel_Name"]]
+            if not vals:
+                raise RuntimeError(f"no {kernel} dispatch in the counter run")
+            res[counter] = (sum(vals) / len(vals), scale)
+        traffic = res["FETCH_SIZE"][0] + res["WRITE_SIZE"][0]
+        return traffic, {"fetch_bytes": res["FETCH_SIZE"][0], "write_bytes": res["WRITE_SIZE"][0],
+                         "fetch_scale": res["FETCH_SIZE"][1], "write_scale": res["WRITE_SIZE"][1]}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+# ------------------------------------------------------------------ learned
+def learned_comparison(s512, n, reps=3):
+    """Sampled splitters vs the learned partitioner on the same input."""
+    import torch
+    dev = torch.device("cuda:0")
+    src = torch.empty(n, dtype=torch.float64, device=dev)
+    work = torch.empty_like(src)
+    out = {}
+    for kind in ("mixture", "uniform"):
+        s512.fill(src, kind, seed=7)
+        times = {"splitters": [], "learned": []}
+        ref = None
+        info = {}
+        for rep in range(reps + 1):
+            for name in ("splitters", "learned"):
+                work.copy_(src)
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                st = s512.sort_learned(work) if name == "learned" else s512.sort_ex(work)
+                e1.record()
+                e1.synchronize()
+                if rep:
+                    times[name].append(e0.elapsed_time(e1))
+                if ref is None:
+                    ref = work.clone()
+                elif not torch.equal(work.view(torch.int64), ref.view(torch.int64)):
+                    raise RuntimeError(f"learned and splitter sorts differ on {kind}")
+                if name == "learned":
+                    info = {"fallback": int(st.learned_fallback), "levels_by_model": int(st.learned_levels),
+                            "max_bucket_ratio": round(float(st.learned_max_bucket_ratio), 3)}
+        sp, le = min(times["splitters"]), min(times["learned"])
+        out[kind] = {"splitters_ms": sp, "learned_ms": le, "speedup": sp / le, **info}
+    del src, work
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box in this setup has one GPU and the sort runs on one device "
               f"(asked for --gpus {args.gpus}).", file=sys.stderr)
         sys.exit(2)
+    # counter runs first, before this process touches the GPU
+    traffic_runs = None
+    if not args.no_traffic:
+        try:
+            traffic_runs = {"uniform_scatter": measure_traffic(args.n, KERNEL_OF_PHASE["scatter"])}
+        except Exception as e:  # reported, not hidden
+            traffic_runs = {"error": repr(e)}
     import torch
     import s512
     dev = torch.device("cuda:0")
@@ -123,11 +227,8 @@ def main():
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
     per_dist_ms = {d: [] for d in dists}
-    phase_ms = {}
-    phase_launch = {}
-    phase_keys = {}
-    levels = {}
-    checked = {}
+    phase_ms, phase_launch, phase_keys = {}, {}, {}
+    levels, checked = {}, {}
     torch.cuda.synchronize()
     wall0 = None
     for step in range(args.warmup + args.steps):
@@ -157,13 +258,20 @@ def main():
     keys = n * len(dists) * args.steps
     value = keys / (total_ms / 1e3)
     key_bytes = 8
-    # dominant kernel: largest device time; algorithmic bytes per key:
-    # count reads each key once, scatter/tile/copy read and write it once.
     bpk = {"count": key_bytes, "scatter": 2 * key_bytes, "tile": 2 * key_bytes, "copy": 2 * key_bytes}
     dom = max((p for p in bpk if phase_launch.get(p)), key=lambda p: phase_ms[p])
     per_launch_bytes = phase_keys[dom] * bpk[dom] / phase_launch[dom]
     per_launch_ms = phase_ms[dom] / phase_launch[dom]
     achieved = per_launch_bytes / (per_launch_ms / 1e3) / 1e9
+    traffic, traffic_detail = None, None
+    if traffic_runs and "uniform_scatter" in traffic_runs and dom == "scatter":
+        traffic, traffic_detail = traffic_runs["uniform_scatter"]
+    elif traffic_runs and "error" in traffic_runs:
+        traffic_detail = traffic_runs["error"]
+    for d in dists:  # free the inputs before the learned comparison
+        pristine[d] = None
+    del work
+    torch.cuda.empty_cache()
     out = {
         "metric": "sorted keys/s + % HBM peak, 256M fp64; learned vs sampled splitters",
         "value": value,
@@ -179,9 +287,10 @@ def main():
         "data": "synthetic (device-generated SplitMix64 / Box-Muller; five distributions)",
         "config": {"workload": f"plain sort, {n} fp64 keys x {len(dists)} distributions ({','.join(dists)})",
                    "keys_per_step": n * len(dists), "classifier": "sampled splitters"},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_launch": per_launch_bytes, "avg_launch_ms": per_launch_ms},
+        "roofline": {"bound": "hbm", "kernel": KERNEL_OF_PHASE[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_detail": traffic_detail, "bytes_per_launch": per_launch_bytes,
+                     "avg_launch_ms": per_launch_ms},
         "effective_hbm_frac": (16.0 * keys / (total_ms / 1e3) / 1e9) / HBM_PEAK_GBS,
         "per_distribution_ms": {d: sum(v) / len(v) for d, v in per_dist_ms.items()},
         "per_distribution_keys_per_s": {d: n / (sum(v) / len(v) / 1e3) for d, v in per_dist_ms.items()},
@@ -189,8 +298,14 @@ def main():
         "phase_ms_per_step": {p: phase_ms[p] / args.steps for p in phase_ms if phase_launch.get(p)},
         "verified": checked,
         "wall_s_timed_region": wall,
+        "learned": None,
         "cpu_baseline": None,
     }
+    if not args.no_learned:
+        try:
+            out["learned"] = learned_comparison(s512, n)
+        except Exception as e:  # reported, not hidden
+            out["learned"] = {"error": repr(e)}
     if not all(checked.values()):
         out["error"] = "verification failed"
     if not args.no_cpu_baseline:
