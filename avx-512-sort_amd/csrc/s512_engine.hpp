@@ -30,6 +30,12 @@ EngineParams default_params(bool wide_keys);
 // Defaults overridden by the non-zero fields of `opt` (may be null).
 EngineParams params_from(const s512_options* opt, bool wide_keys);
 
+// Workload kinds of the autotuned defaults table (s512_tune_table.inc).
+enum TuneKind { TUNE_I32 = 0, TUNE_F64 = 1, TUNE_KV = 2, TUNE_SEG_I32 = 3, TUNE_SEG_F64 = 4 };
+// Defaults, then the autotuned entry for (kind, n keys), then the non-zero
+// fields of `opt` (explicit options always win).
+EngineParams params_for(const s512_options* opt, int kind, uint64_t n);
+
 template <class T, bool KV>
 s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const EngineParams& prm, s512_stats* stats,
                         bool profile);

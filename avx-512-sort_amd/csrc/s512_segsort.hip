@@ -370,7 +370,8 @@ static s512_status seg_finish(T* a, T* b, const uint64_t* off, size_t nseg, size
 }
 
 template <class T>
-static s512_status segsort_dev(T* a, const uint64_t* off, size_t nseg, hipStream_t st, s512_stats* stats) {
+static s512_status segsort_dev(T* a, const uint64_t* off, size_t nseg, hipStream_t st, s512_stats* stats,
+                               const s512_options* opt = nullptr) {
     if (stats) memset(stats, 0, sizeof(*stats));
     if (nseg == 0) return S512_OK;
     if (!a || !off) { set_error("null pointer"); return S512_ERR_INVALID; }
@@ -389,7 +390,7 @@ static s512_status segsort_dev(T* a, const uint64_t* off, size_t nseg, hipStream
                                  (unsigned long long)h[0], (unsigned long long)h[1]); return S512_ERR_INVALID; }
     const size_t nkeys = h[1];  // keys addressed: [0, off[nseg])
     if (nkeys >= (1ull << 32)) { set_error("%zu keys exceed 2^32-1", nkeys); return S512_ERR_TOO_LARGE; }
-    const EngineParams prm = default_params(sizeof(T) == 8);
+    const EngineParams prm = params_for(opt, sizeof(T) == 8 ? TUNE_SEG_F64 : TUNE_SEG_I32, nkeys);
     int ncls;
     const uint32_t* caps = tile_caps<T>(&ncls);
     const SegWs probe = seg_layout<T>(nullptr, nseg, nkeys, prm, ncls, caps);
@@ -419,7 +420,7 @@ template <class T>
 static s512_status plan_create(SegPlan* p) {
     s512_status rc = current_device(&p->dev);
     if (rc) return rc;
-    const EngineParams prm = default_params(sizeof(T) == 8);
+    const EngineParams prm = params_for(nullptr, sizeof(T) == 8 ? TUNE_SEG_F64 : TUNE_SEG_I32, p->nkeys);
     int ncls;
     const uint32_t* caps = tile_caps<T>(&ncls);
     const SegWs probe = seg_layout<T>(nullptr, p->nseg, p->nkeys, prm, ncls, caps);
@@ -459,7 +460,7 @@ static s512_status plan_launch(SegPlan* p, hipStream_t st, s512_stats* stats) {
     S512_HIP(hipGraphLaunch(p->exec, st));
     DeviceState& ds = device_state(p->dev);
     std::lock_guard<std::mutex> lk(ds.mu);
-    const EngineParams prm = default_params(sizeof(T) == 8);
+    const EngineParams prm = params_for(nullptr, sizeof(T) == 8 ? TUNE_SEG_F64 : TUNE_SEG_I32, p->nkeys);
     return seg_finish<T>((T*)p->keys, (T*)p->scratch, p->off, p->nseg, p->nkeys, p->w, st, prm, ds, stats);
 }
 
@@ -501,6 +502,14 @@ extern "C" s512_status s512_segsort_i32_dev(int32_t* d_keys, const uint64_t* d_o
 extern "C" s512_status s512_segsort_f64_dev(double* d_keys, const uint64_t* d_offsets, size_t nseg,
                                             s512_stats* stats, void* stream) {
     return segsort_dev<double>(d_keys, d_offsets, nseg, (hipStream_t)stream, stats);
+}
+extern "C" s512_status s512_segsort_i32_dev_ex(int32_t* d_keys, const uint64_t* d_offsets, size_t nseg,
+                                               const s512_options* opt, s512_stats* stats, void* stream) {
+    return segsort_dev<int32_t>(d_keys, d_offsets, nseg, (hipStream_t)stream, stats, opt);
+}
+extern "C" s512_status s512_segsort_f64_dev_ex(double* d_keys, const uint64_t* d_offsets, size_t nseg,
+                                               const s512_options* opt, s512_stats* stats, void* stream) {
+    return segsort_dev<double>(d_keys, d_offsets, nseg, (hipStream_t)stream, stats, opt);
 }
 extern "C" s512_status s512_segsort_i32(int32_t* keys, const uint64_t* offsets, size_t nseg) {
     return segsort_host<int32_t>(keys, offsets, nseg);

@@ -571,6 +571,50 @@ EngineParams params_from(const s512_options* opt, bool wide) {
     return p;
 }
 
+// ---------------------------------------------------------------- tuned defaults
+// One autotuned entry per workload kind and size class: n in [2^lg_lo, 2^lg_hi).
+struct TuneEntry {
+    int kind, lg_lo, lg_hi;
+    int count_shape, scatter_shape, ids, tile_plan;
+    uint32_t target_leaf;
+    int max_logf;
+};
+#include "s512_tune_table.inc"  // kTune[]: generated by avx-512-sort_amd/s512_tune.py
+
+static void apply_tuned(EngineParams& p, int kind, uint64_t n) {
+    int lg = 0;
+    while (lg < 63 && (1ull << (lg + 1)) <= n) ++lg;  // floor(log2 n)
+    for (const TuneEntry& e : kTune) {
+        if (e.kind != kind || lg < e.lg_lo || lg >= e.lg_hi) continue;
+        p.count_shape = e.count_shape;
+        p.scatter_shape = e.scatter_shape;
+        p.ids = e.ids != 0;
+        p.tile_plan = e.tile_plan;
+        p.target_leaf = e.target_leaf;
+        p.max_logf = e.max_logf;
+        return;
+    }
+}
+
+EngineParams params_for(const s512_options* opt, int kind, uint64_t n) {
+    EngineParams p = default_params(kind == TUNE_F64 || kind == TUNE_KV || kind == TUNE_SEG_F64);
+    apply_tuned(p, kind, n);
+    if (!opt) return p;
+    const EngineParams d = p;
+    p = params_from(opt, false);
+    // params_from starts from the untuned defaults: keep the tuned value of
+    // every knob the caller left at zero
+    if (!opt->target_leaf) p.target_leaf = d.target_leaf;
+    if (!opt->max_fanout_log2) p.max_logf = d.max_logf;
+    if (!opt->scatter_variant) {
+        p.scatter_shape = d.scatter_shape;
+        p.ids = d.ids;
+    }
+    if (!opt->count_variant) p.count_shape = d.count_shape;
+    if (!opt->tile_variant) p.tile_plan = d.tile_plan;
+    return p;
+}
+
 // ---------------------------------------------------------------- host pointers
 // Run `fn` on device memory: in place for device pointers, through a staging
 // copy for host pointers.
@@ -641,21 +685,21 @@ extern "C" s512_status s512_sort_i32_dev_ex(int32_t* d_a, size_t n, const s512_o
                                             void* stream) {
     if (n > 1 && !d_a) { set_error("null array"); return S512_ERR_INVALID; }
     if (s512_status rc = check_opts(opt)) return rc;
-    return sort_engine<int32_t, false>(d_a, nullptr, n, (hipStream_t)stream, params_from(opt, false), stats,
+    return sort_engine<int32_t, false>(d_a, nullptr, n, (hipStream_t)stream, params_for(opt, TUNE_I32, n), stats,
                                        opt && opt->profile);
 }
 extern "C" s512_status s512_sort_f64_dev_ex(double* d_a, size_t n, const s512_options* opt, s512_stats* stats,
                                             void* stream) {
     if (n > 1 && !d_a) { set_error("null array"); return S512_ERR_INVALID; }
     if (s512_status rc = check_opts(opt)) return rc;
-    return sort_engine<double, false>(d_a, nullptr, n, (hipStream_t)stream, params_from(opt, true), stats,
+    return sort_engine<double, false>(d_a, nullptr, n, (hipStream_t)stream, params_for(opt, TUNE_F64, n), stats,
                                       opt && opt->profile);
 }
 extern "C" s512_status s512kv_sort_i32_dev_ex(int32_t* d_k, int32_t* d_v, size_t n, const s512_options* opt,
                                               s512_stats* stats, void* stream) {
     if (n > 1 && (!d_k || !d_v)) { set_error("null array"); return S512_ERR_INVALID; }
     if (s512_status rc = check_opts(opt)) return rc;
-    return sort_engine<int32_t, true>(d_k, d_v, n, (hipStream_t)stream, params_from(opt, true), stats,
+    return sort_engine<int32_t, true>(d_k, d_v, n, (hipStream_t)stream, params_for(opt, TUNE_KV, n), stats,
                                       opt && opt->profile);
 }
 extern "C" s512_status s512_sort_i32_dev(int32_t* d_a, size_t n, void* stream) {

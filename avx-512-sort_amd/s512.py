@@ -78,6 +78,8 @@ _SIGS = {
     "s512_segsort_f64": ([_vp, _vp, _c_size], ctypes.c_int),
     "s512_segsort_i32_dev": ([_vp, _vp, _c_size, _vp, _vp], ctypes.c_int),
     "s512_segsort_f64_dev": ([_vp, _vp, _c_size, _vp, _vp], ctypes.c_int),
+    "s512_segsort_i32_dev_ex": ([_vp, _vp, _c_size, _vp, _vp, _vp], ctypes.c_int),
+    "s512_segsort_f64_dev_ex": ([_vp, _vp, _c_size, _vp, _vp, _vp], ctypes.c_int),
     "s512_segsort_plan_create": ([ctypes.c_int, _vp, _vp, _c_size, _c_size, ctypes.POINTER(_vp)], ctypes.c_int),
     "s512_segsort_plan_launch": ([_vp, _vp, _vp], ctypes.c_int),
     "s512_segsort_plan_destroy": ([_vp], ctypes.c_int),
@@ -328,16 +330,19 @@ def _offsets_ok(keys, offsets):
         raise ValueError("offsets holds nseg + 1 entries")
 
 
-def segsort(keys, offsets, stats=False):
+def segsort(keys, offsets, stats=False, options=None):
     """Sort every segment keys[offsets[i]:offsets[i+1]] independently, in place."""
     k = _kind(keys)
     _offsets_ok(keys, offsets)
     nseg = _n(offsets) - 1
     if _dev(keys):
         st = Stats()
-        _check(getattr(lib, f"s512_segsort_{k}_dev")(_ptr(keys), _ptr(offsets), nseg, ctypes.byref(st),
-                                                     _stream_of(keys)))
+        _check(getattr(lib, f"s512_segsort_{k}_dev_ex")(_ptr(keys), _ptr(offsets), nseg,
+                                                        ctypes.byref(options) if options is not None else None,
+                                                        ctypes.byref(st), _stream_of(keys)))
         return st if stats else keys
+    if options is not None:
+        raise ValueError("options apply to device arrays")
     _check(getattr(lib, f"s512_segsort_{k}")(_ptr(keys), _ptr(offsets), nseg))
     return keys
 

@@ -102,7 +102,9 @@ s512_status s512_merge_inplace_i32_dev(int32_t* d_a, size_t size, size_t center,
 s512_status s512_merge_inplace_f64_dev(double* d_a, size_t size, size_t center, void* stream);
 
 /* ------------------------------------------------------ options and stats */
-/* Tuning knobs and per-call observability.  Zero fields select defaults.  */
+/* Tuning knobs and per-call observability.  Zero fields select the defaults:
+ * the autotuned table compiled into the library (avx-512-sort_amd/s512_tune.py
+ * writes it per dtype and size class), which an explicit field overrides.  */
 enum {
     S512_PHASE_SAMPLE = 0,      /* draw splitter samples                    */
     S512_PHASE_SAMPLE_SORT = 1, /* bitonic sort of the samples              */
@@ -212,6 +214,11 @@ s512_status s512_segsort_i32_dev(int32_t* d_keys, const uint64_t* d_offsets, siz
                                  void* stream);
 s512_status s512_segsort_f64_dev(double* d_keys, const uint64_t* d_offsets, size_t nseg, s512_stats* stats,
                                  void* stream);
+/* Same with tuning options (NULL or zero fields: the autotuned defaults). */
+s512_status s512_segsort_i32_dev_ex(int32_t* d_keys, const uint64_t* d_offsets, size_t nseg, const s512_options* opt,
+                                    s512_stats* stats, void* stream);
+s512_status s512_segsort_f64_dev_ex(double* d_keys, const uint64_t* d_offsets, size_t nseg, const s512_options* opt,
+                                    s512_stats* stats, void* stream);
 /* Reusable plan: the kernel sequence for fixed (d_keys, d_offsets, nseg,
  * nkeys = length of d_keys) captured once in a hipGraph, with its own
  * workspace.  Launch replays the graph on `stream` (the offsets and keys may

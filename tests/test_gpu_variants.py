@@ -95,3 +95,31 @@ def test_variant_out_of_range(s512, dev):
         setattr(o, field, bad)
         with pytest.raises(s512.S512Error):
             s512.sort_ex(t, o)
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.float64])
+@pytest.mark.parametrize("n", [9000, 100_000, 300_000, 1_000_000, 5_000_000, 9_000_000])
+def test_tuned_defaults_every_class(s512, orc, dev, dtype, n):
+    """Default options take the autotuned table entry of n's size class."""
+    a = _input(orc, dtype, "normal")
+    a = np.concatenate([a] * (n // len(a) + 1))[:n] if n > len(a) else a[:n].copy()
+    got, _ = _run(s512, dev, a)
+    assert_keys_equal(got, orc.closed_sort(a.copy()), f"tuned defaults n={n}")
+
+
+def test_tuned_defaults_kv_and_segmented(s512, orc, dev):
+    k = (orc.gen_i32_full(2_000_000, 9) & 0xFFFFFF).astype(np.int32)
+    v = np.arange(k.size, dtype=np.int32)
+    tk, tv = to_dev(k, dev), to_dev(v, dev)
+    s512.kv_sort(tk, tv)
+    gk, gv = to_host(tk), to_host(tv)
+    assert np.array_equal(gk, orc.closed_sort(k.copy()))
+    assert np.array_equal(orc.pair_multiset(gk, gv), orc.pair_multiset(k, v))
+    import torch
+    off = s512.lognormal_offsets(2000, seed=4)
+    keys = orc.gen_f64("normal", int(off[-1]), 4)
+    tk = to_dev(keys, dev)
+    s512.segsort(tk, torch.from_numpy(off.view(np.int64)).to(dev), options=s512.Options())
+    got = to_host(tk)
+    for i in range(len(off) - 1):
+        assert np.array_equal(got[off[i]:off[i + 1]], np.sort(keys[off[i]:off[i + 1]])), i
