@@ -19,6 +19,7 @@ struct EngineParams {
     uint32_t train_steps;    // learned: Adam steps
     float learn_rate;        // learned: Adam step size
     float imbalance;         // learned: fall back when a level-1 bucket exceeds this x mean
+    float max_loss;          // learned: reject a network whose training loss (MSE on the CDF) is above
     uint64_t seed;           // sample positions
     int scatter_shape;       // k_scatter workgroup shape (0 .. SCATTER_SHAPES-1)
     bool ids;                // count stores each key's bucket; scatter reads it

@@ -9,10 +9,11 @@
 
 namespace s512 {
 
-// State of the learned partitioner across the level loop.
+// State of the learned partitioner across the level loop: the model places
+// the splitters of the root level (2^lf1 buckets) and of its buckets.
 struct LevelState {
     bool use_learned = false;
-    uint32_t lf1 = 0, lf2 = 0;
+    uint32_t lf1 = 0;
 };
 
 // The partition level loop over an initial list of segments of the arrays
@@ -21,7 +22,7 @@ struct LevelState {
 template <class T, bool KV>
 s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vector<Seg> segs, hipStream_t st,
                        const EngineParams& prm, s512_stats* stats, Profiler& pf, DeviceState& ds,
-                       const LearnedTable& ltab, LevelState& lstate);
+                       const LearnedModelDev& model, LevelState& lstate);
 
 // Capacities (keys) of the tile classes for plain keys of type T.
 template <class T>

@@ -160,35 +160,17 @@ __device__ __forceinline__ uint32_t lower_bound_lds(const K* srt, uint32_t n, K 
     return lo;
 }
 
-// Load a segment's classifier into LDS: splitters + guide table, or (learned
-// segments) the tabulated model in the guide table's place.
-constexpr int LDS_TABLE_WORDS = GMAX > LEARN_G + 1 ? GMAX : LEARN_G + 1;
+// Load a segment's classifier into LDS: splitters + guide table.
+constexpr int LDS_TABLE_WORDS = GMAX;
 template <class K, int NT>
 __device__ __forceinline__ Guide load_classifier(const PartSeg& s, const K* trees, const uint32_t* guides, K* lsrt,
-                                                 uint32_t* lguide, const LearnedTable& lt) {
-    if (s.mode) {
-        for (uint32_t j = threadIdx.x; j <= LEARN_G; j += NT) lguide[j] = __float_as_uint(lt.tab[j]);
-        return Guide{0, 0, 0, 0, 0, 0};
-    }
+                                                 uint32_t* lguide) {
     const uint32_t F = 1u << s.logf, G = guide_words(F);
     for (uint32_t j = threadIdx.x; j < F; j += NT) lsrt[j] = trees[s.tree_base + j];
     for (uint32_t j = threadIdx.x; j < G; j += NT) lguide[j] = guides[s.guide_base + j];
     return make_guide(s.kmin, s.shift, F, s.flags);
 }
 
-// Bucket of key x in segment s (learned segments use only even buckets).
-template <class K>
-__device__ __forceinline__ uint32_t classify_seg(const PartSeg& s, const K* lsrt, const uint32_t* lguide,
-                                                 const Guide& g, const LearnedTable& lt, K x) {
-    if (s.mode) {
-        const uint32_t fb = learned_fine_bucket((const float*)lguide, lt.kmin, lt.inv_range, lt.nb_log, x);
-        int b = (int)(fb >> s.lshift) - (int)s.lbase;
-        const int fmax = (1 << s.logf) - 1;
-        b = b < 0 ? 0 : (b > fmax ? fmax : b);
-        return 2u * (uint32_t)b;
-    }
-    return classify(lsrt, lguide, g, x);
-}
 
 // Buckets of NB keys of one segment at once.  Each stage issues its LDS reads
 // for all NB keys before any result is used and no stage branches per key, so
@@ -260,17 +242,6 @@ __device__ __forceinline__ void classify_batch(const K* srt, const uint32_t* gd,
     }
 }
 
-template <class K, int NB>
-__device__ __forceinline__ void classify_seg_batch(const PartSeg& s, const K* lsrt, const uint32_t* lguide,
-                                                   const Guide& g, const LearnedTable& lt, const K (&x)[NB],
-                                                   uint32_t (&bk)[NB]) {
-    if (s.mode) {
-#pragma unroll
-        for (int u = 0; u < NB; ++u) bk[u] = classify_seg(s, lsrt, lguide, g, lt, x[u]);
-    } else {
-        classify_batch<K, NB>(lsrt, lguide, g, x, bk);
-    }
-}
 
 // ---------------------------------------------------------------- tile kernels
 // How keys of type T enter and leave a network ordered by policy Pol.
@@ -447,12 +418,36 @@ __global__ __launch_bounds__(256) void k_sample(const PartSeg* ps, const uint32_
     using Tr = KeyTraits<T>;
     for (uint32_t w = blockIdx.x; w < *nseg; w += gridDim.x) {
         const PartSeg s = ps[w];
+        if (s.mode) continue;  // learned segment: k_model_splitters writes its splitters
         const T* src = (s.buf ? b : a) + s.begin;
         typename Tr::K* out = samples + s.sample_base;
         for (uint32_t i = threadIdx.x; i < s.nsample; i += blockDim.x) {
             const uint64_t h = mix64(seed ^ (s.begin * 0x632BE59BD9B4E019ull), i);
             const uint64_t pos = __umul64hi(h, s.size);  // uniform in [0, size)
             out[i] = Tr::enc(src[pos]);
+        }
+    }
+}
+
+// Learned segments (mode 1): the splitters are quantiles of the learned CDF
+// model (s512_learned.hpp) instead of a sorted random sample.  Quantile range
+// of a segment: the learned root (lshift = 0) covers [0, 1); bucket j
+// (= lbase) of a 2^lshift-way learned root covers [j, j + 1) / 2^lshift.
+// Written as a sorted pseudo-sample of F keys -- the F - 1 quantiles, then the
+// range's upper key -- which k_build_tree turns into splitters exactly as it
+// does a sample of F keys.
+template <class K>
+__global__ __launch_bounds__(256) void k_model_splitters(const PartSeg* ps, const uint32_t* nseg,
+                                                         LearnedModelDev model, K* samples) {
+    for (uint32_t w = blockIdx.x; w < *nseg; w += gridDim.x) {
+        const PartSeg s = ps[w];
+        if (!s.mode) continue;
+        const uint32_t F = 1u << s.logf;
+        const double den = (double)(1ull << s.lshift);
+        const double q0 = (double)s.lbase / den, q1 = (double)(s.lbase + 1) / den;
+        for (uint32_t t = threadIdx.x; t < F; t += blockDim.x) {
+            const double q = t + 1 < F ? q0 + (q1 - q0) * (double)(t + 1) / (double)F : q1;
+            samples[s.sample_base + t] = (K)model_quantile(model.key, model.cdf, q);
         }
     }
 }
@@ -599,7 +594,7 @@ __global__ __launch_bounds__(FMAX) void k_build_tree(PartSeg* ps, const uint32_t
     __shared__ TreeLds L;
     for (uint32_t w = blockIdx.x; w < *nseg; w += gridDim.x) {
         PartSeg& s = ps[w];
-        if (s.mode || s.nsample == 0) continue;  // learned segment: no splitters
+        if (s.nsample == 0) continue;
         build_tree_body<K>(s, samples + s.sample_base, trees, guides, srt, L);
         __syncthreads();
     }
@@ -659,7 +654,7 @@ template <class T, int CNT_P, int CNT_U, int NB>
 __global__ __launch_bounds__(CNT_P) void k_count(const PartSeg* ps, const ChunkRef* cr, const uint32_t* nchunk,
                                                  const T* a, const T* b, const typename KeyTraits<T>::K* trees,
                                                  const uint32_t* guides, uint32_t* hist, uint32_t* special,
-                                                 LearnedTable lt, uint16_t* ids) {
+                                                 uint16_t* ids) {
     using Tr = KeyTraits<T>;
     using K = typename Tr::K;
     constexpr int STEP = CNT_P * CNT_U;
@@ -672,7 +667,7 @@ __global__ __launch_bounds__(CNT_P) void k_count(const PartSeg* ps, const ChunkR
     const PartSeg s = ps[c.seg];
     const uint32_t F = 1u << s.logf, B = 2 * F;
     const uint32_t t = threadIdx.x;
-    const Guide g = load_classifier<K, CNT_P>(s, trees, guides, lsrt, lguide, lt);
+    const Guide g = load_classifier<K, CNT_P>(s, trees, guides, lsrt, lguide);
     for (uint32_t j = t; j < B; j += CNT_P) lh[j] = 0;
     __syncthreads();
     const T* src = (s.buf ? b : a) + s.begin;
@@ -698,7 +693,7 @@ __global__ __launch_bounds__(CNT_P) void k_count(const PartSeg* ps, const ChunkR
                 uint32_t kb[NB];
 #pragma unroll
                 for (int u = 0; u < NB; ++u) xb[u] = x[u0 + u];
-                classify_seg_batch<K, NB>(s, lsrt, lguide, g, lt, xb, kb);
+                classify_batch<K, NB>(lsrt, lguide, g, xb, kb);
 #pragma unroll
                 for (int u = 0; u < NB; ++u) bk[u0 + u] = kb[u];
             }
@@ -713,7 +708,7 @@ __global__ __launch_bounds__(CNT_P) void k_count(const PartSeg* ps, const ChunkR
             for (int u = 0; u < CNT_U; ++u) {
                 const uint64_t i = base + (uint64_t)u * CNT_P + t;
                 if (i < hi) {
-                    const uint32_t bk = classify_seg(s, lsrt, lguide, g, lt, x[u]);
+                    const uint32_t bk = classify(lsrt, lguide, g, x[u]);
                     odd |= nan_or_negzero(x[u]);
                     atomicAdd(&lh[bk], 1u);
                     if (oid) oid[i] = (uint16_t)bk;
@@ -743,10 +738,9 @@ __global__ __launch_bounds__(CNT_P) void k_count(const PartSeg* ps, const ChunkR
 template <class T>
 inline void launch_count(int shape, uint32_t grid, hipStream_t st, const PartSeg* ps, const ChunkRef* cr,
                          const uint32_t* nchunk, const T* a, const T* b, const typename KeyTraits<T>::K* trees,
-                         const uint32_t* guides, uint32_t* hist, uint32_t* special, const LearnedTable& lt,
-                         uint16_t* ids) {
+                         const uint32_t* guides, uint32_t* hist, uint32_t* special, uint16_t* ids) {
 #define S512_COUNT(PP, UU, NN) \
-    k_count<T, PP, UU, NN><<<grid, PP, 0, st>>>(ps, cr, nchunk, a, b, trees, guides, hist, special, lt, ids)
+    k_count<T, PP, UU, NN><<<grid, PP, 0, st>>>(ps, cr, nchunk, a, b, trees, guides, hist, special, ids)
     switch (shape) {
         case 1: S512_COUNT(256, 16, 4); break;
         case 2: S512_COUNT(256, 8, 8); break;
@@ -873,7 +867,7 @@ template <class T, bool KV, bool IDS, int P, int E, bool WC>
 __global__ __launch_bounds__(P) void k_scatter(const PartSeg* ps, const ChunkRef* cr, T* a, T* b, int32_t* va,
                                                int32_t* vb, const typename KeyTraits<T>::K* trees,
                                                const uint32_t* guides, const uint32_t* hist, const uint64_t* bstart,
-                                               LearnedTable lt, const uint32_t* nchunk, const uint16_t* ids) {
+                                               const uint32_t* nchunk, const uint16_t* ids) {
     using Tr = KeyTraits<T>;
     using K = typename Tr::K;
     using L = ScatterLds<T, KV, IDS, P, E, WC>;
@@ -890,7 +884,7 @@ __global__ __launch_bounds__(P) void k_scatter(const PartSeg* ps, const ChunkRef
     const uint32_t B = 2u << s.logf;
     const uint32_t t = threadIdx.x, lane = t & 63u, wid = t >> 6;
     Guide g{0, 0, 0, 0, 0, 0};
-    if constexpr (!IDS) g = load_classifier<K, P>(s, trees, guides, sm.lsrt, sm.lguide, lt);
+    if constexpr (!IDS) g = load_classifier<K, P>(s, trees, guides, sm.lsrt, sm.lguide);
     const uint32_t* h = hist + s.hist_base + (uint64_t)c.idx * B;
     for (uint32_t j = t; j < BMAX; j += P) {
         sm.cursor[j] = j < B ? (uint32_t)bstart[s.bkt_base + j] + h[j] : 0u;
@@ -943,13 +937,13 @@ __global__ __launch_bounds__(P) void k_scatter(const PartSeg* ps, const ChunkRef
                 uint32_t kb[CLS_NB];
 #pragma unroll
                 for (int u = 0; u < CLS_NB; ++u) kx[u] = Tr::enc(cur.x[e0 + u]);
-                classify_seg_batch<K, CLS_NB>(s, sm.lsrt, sm.lguide, g, lt, kx, kb);
+                classify_batch<K, CLS_NB>(sm.lsrt, sm.lguide, g, kx, kb);
 #pragma unroll
                 for (int u = 0; u < CLS_NB; ++u) bk[e0 + u] = kb[u];
             }
         } else {
 #pragma unroll
-            for (int e = 0; e < E; ++e) bk[e] = classify_seg(s, sm.lsrt, sm.lguide, g, lt, Tr::enc(cur.x[e]));
+            for (int e = 0; e < E; ++e) bk[e] = classify(sm.lsrt, sm.lguide, g, Tr::enc(cur.x[e]));
         }
         if (cnt == ROUND) {
             wave_rank_many<E>(sm.rh, sm.dummy, bk, lr);
@@ -1062,10 +1056,10 @@ template <class T, bool KV>
 inline void launch_scatter(int shape, bool use_ids, uint32_t grid, hipStream_t st, const PartSeg* ps,
                            const ChunkRef* cr, T* a, T* b, int32_t* va, int32_t* vb,
                            const typename KeyTraits<T>::K* trees, const uint32_t* guides, const uint32_t* hist,
-                           const uint64_t* bstart, const LearnedTable& lt, const uint32_t* nchunk,
+                           const uint64_t* bstart, const uint32_t* nchunk,
                            const uint16_t* ids) {
 #define S512_SCATTER(I, PP, EE, W)                                                                              \
-    k_scatter<T, KV, I, PP, EE, W><<<grid, PP, 0, st>>>(ps, cr, a, b, va, vb, trees, guides, hist, bstart, lt, nchunk, \
+    k_scatter<T, KV, I, PP, EE, W><<<grid, PP, 0, st>>>(ps, cr, a, b, va, vb, trees, guides, hist, bstart, nchunk, \
                                                         ids)
     if (use_ids) {
         switch (shape) {
@@ -1160,9 +1154,9 @@ struct PlanLists {
                 pl.copy[o + p] = TileSeg{begin + pb, (uint32_t)((sz - pb) < COPY_PIECE ? (sz - pb) : COPY_PIECE), 1u};
             }
         } else {
-            // a bucket of a learned level with a finer level below keeps the
-            // learned classifier: aux = (bucket index << 1) | 1
-            const uint32_t aux = (s.mode && s.lshift) ? ((((b >> 1) + s.lbase) << 1) | 1u) : 0u;
+            // a bucket of the learned root level is partitioned by the model
+            // again: aux = (bucket index << 1) | 1
+            const uint32_t aux = (s.mode && s.lshift == 0) ? (((b >> 1) << 1) | 1u) : 0u;
             pl.next[o] = Seg{begin, sz, buf, aux};
         }
     }

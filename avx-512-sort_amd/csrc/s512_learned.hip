@@ -1,18 +1,20 @@
-// s512_learned.hip -- GPU training and tabulation of the learned partitioner's
-// CDF model (see s512_learned.hpp for the model).  No reference counterpart.
+// s512_learned.hip -- GPU training of the learned partitioner's CDF model and
+// its knots (see s512_learned.hpp for the model).  No reference counterpart.
 //
 //   k_learn_sample   gather ~1% of the keys (hashed positions), encoded
 //   (plain sort)     the sample is sorted by this library's own sort engine
-//   k_learn_prep     features u and targets (rank+0.5)/M of M evenly spaced
+//   k_learn_prep     features u and targets (rank+0.5)/ns of M evenly spaced
 //                    sample points; parameters initialised so that the net is
 //                    the piecewise-linear interpolation through H+1 sample
 //                    quantiles
-//   k_mlp_step       one full-batch Adam step: per point forward + backward,
-//                    per-workgroup partial gradients (fixed summation order),
-//                    the last workgroup to finish sums them in block order and
-//                    updates the parameters (deterministic, no spin-waits)
-//   k_tabulate       model on LEARN_G+1 grid points, clamped to [0,1], made
-//                    non-decreasing by a running maximum
+//   k_mlp_train      every Adam step in one workgroup (one point per thread):
+//                    forward + backward, the gradient summed in a fixed order
+//                    (deterministic), the update; the final full-batch loss
+//   k_knots          knot keys at equal sample mass, the network's CDF there
+//                    kept within the sample's band
+//   k_knot_scan      running maximum: the knot CDF values made non-decreasing
+// k_mlp_step is the same gradient spread over many workgroups, for the
+// gradient check of s512_mlp_loss_grad_f32 on more points than one workgroup.
 #include "s512_learned.hpp"
 #include "s512_kernels.hpp"
 #include "s512_runtime.hpp"
@@ -72,111 +74,168 @@ struct TrainBufs {
     float* loss;    // last full-batch loss
 };
 
-__device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
-    return v;
-}
+// Scratch of one workgroup's gradient computation: per-point activations
+// and deltas laid out [unit][point] (row stride TRAIN_P + 1: the row sums
+// below read one column per lane, on distinct banks).
+struct MlpLds {
+    float sh1[MLP_H][TRAIN_P + 1];  // h1
+    float sh2[MLP_H][TRAIN_P + 1];  // h2 (its ramp is active iff 0 < h2 < 1)
+    float sdd[MLP_H][TRAIN_P + 1];  // layer-1 delta (ramp mask applied)
+    float sdy[TRAIN_P];             // dLoss/dy
+    float sx[TRAIN_P];              // feature
+    float se2[TRAIN_P];             // squared error
+};
 
-// grad_only: the last block writes the summed gradient to gout instead of
-// updating the parameters.
-__global__ __launch_bounds__(TRAIN_P) void k_mlp_step(TrainBufs tb, const float* u, const float* tg, uint32_t M,
-                                                      float lr, int grad_only, float* gout) {
+// Gradient of sum_i (y(u_i) - t_i)^2 / M over this workgroup's points (one per
+// thread; `valid` false for a thread without one): out[i] for parameter i,
+// out[MLP_NP] = the workgroup's sum of squared errors.  p: parameters, w2 =
+// their squared layer-2 weights (both in LDS).  Every sum over points runs
+// over the points in order, so the result does not depend on scheduling.
+// Ends with a barrier.  Inlined into its kernels so that the LDS arrays are
+// addressed as LDS (a call would pass them as generic pointers).
+__device__ __forceinline__ void mlp_block_grad(const float* p, const float* w2, float x, float target, bool valid, uint32_t M,
+                               MlpLds& L, float* out) {
     constexpr int H = MLP_H;
-    constexpr int NW = TRAIN_P / 64;
-    constexpr int NSMALL = 4 * H + 1;  // p1, b1, b2, p3 gradients and b3
-    __shared__ float p[MLP_NP];
-    __shared__ float w2[H * H];
-    __shared__ float sh1[H][TRAIN_P + 1];  // h1 per unit and point
-    __shared__ float sd2[H][TRAIN_P + 1];  // layer-2 pre-activation, then its delta
-    __shared__ float wred[NW][NSMALL + 1];
-    __shared__ uint32_t last;
-    const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid >> 6;
-    for (int i = tid; i < MLP_NP; i += TRAIN_P) p[i] = tb.params[i];
-    __syncthreads();
-    for (int i = tid; i < H * H; i += TRAIN_P) w2[i] = p[MLP_OFF_P2 + i] * p[MLP_OFF_P2 + i];
-    __syncthreads();
-    const uint32_t pt = blockIdx.x * TRAIN_P + tid;
-    const bool valid = pt < M;
-    const float x = valid ? u[pt] : 0.0f, target = valid ? tg[pt] : 0.0f;
-    // forward, layer 1 (ramp-derivative masks kept as bits)
+    const uint32_t tid = threadIdx.x;
+    // forward
     float h1[H];
-    uint32_t m1 = 0;
 #pragma unroll
     for (int j = 0; j < H; ++j) {
-        const float z = fmaf(p[j] * p[j], x, p[MLP_OFF_B1 + j]);
-        h1[j] = ramp01(z);
-        m1 |= (z > 0.0f && z < 1.0f) ? (1u << j) : 0u;
-        sh1[j][tid] = h1[j];
+        h1[j] = ramp01(fmaf(p[j] * p[j], x, p[MLP_OFF_B1 + j]));
+        L.sh1[j][tid] = h1[j];
     }
-    // forward, layer 2 and output
+    // (the unit loops stay rolled: unrolled, the compiler hoists all H*H
+    // weight loads and runs out of registers)
     float y = p[MLP_OFF_B3];
+#pragma unroll 2
     for (int k = 0; k < H; ++k) {
         float z = p[MLP_OFF_B2 + k];
 #pragma unroll
         for (int j = 0; j < H; ++j) z = fmaf(w2[k * H + j], h1[j], z);
-        sd2[k][tid] = z;
-        y = fmaf(p[MLP_OFF_P3 + k] * p[MLP_OFF_P3 + k], ramp01(z), y);
+        const float h2 = ramp01(z);
+        L.sh2[k][tid] = h2;
+        y = fmaf(p[MLP_OFF_P3 + k] * p[MLP_OFF_P3 + k], h2, y);
     }
     const float e = valid ? y - target : 0.0f;
     const float dy = 2.0f * e / (float)M;
-    // backward, layer 2: delta per unit; reduce the b2 and p3 contributions
+    L.sdy[tid] = dy;
+    L.sx[tid] = x;
+    L.se2[tid] = e * e;
+    // backward to layer 1 (per point)
     float d1[H];
 #pragma unroll
     for (int j = 0; j < H; ++j) d1[j] = 0.0f;
+#pragma unroll 2
     for (int k = 0; k < H; ++k) {
-        const float z = sd2[k][tid];
         const float p3 = p[MLP_OFF_P3 + k];
-        const float d2 = (z > 0.0f && z < 1.0f) ? dy * p3 * p3 : 0.0f;
-        sd2[k][tid] = d2;
+        const float h2 = L.sh2[k][tid];
+        const float d2 = (h2 > 0.0f && h2 < 1.0f) ? dy * p3 * p3 : 0.0f;
 #pragma unroll
         for (int j = 0; j < H; ++j) d1[j] = fmaf(d2, w2[k * H + j], d1[j]);
-        float gv = wave_sum(d2);
-        if (lane == 0) wred[wid][2 * H + k] = gv;                       // b2
-        gv = wave_sum(dy * 2.0f * p3 * ramp01(z));
-        if (lane == 0) wred[wid][3 * H + k] = gv;                       // p3
     }
-    // backward, layer 1
 #pragma unroll
-    for (int j = 0; j < H; ++j) {
-        const float dd = ((m1 >> j) & 1u) ? d1[j] : 0.0f;
-        float gv = wave_sum(dd * 2.0f * p[j] * x);
-        if (lane == 0) wred[wid][j] = gv;                               // p1
-        gv = wave_sum(dd);
-        if (lane == 0) wred[wid][H + j] = gv;                           // b1
-    }
-    float gv = wave_sum(dy);
-    if (lane == 0) wred[wid][4 * H] = gv;                               // b3
-    gv = wave_sum(e * e);
-    if (lane == 0) wred[wid][NSMALL] = gv;                              // loss
+    for (int j = 0; j < H; ++j) L.sdd[j][tid] = (h1[j] > 0.0f && h1[j] < 1.0f) ? d1[j] : 0.0f;
     __syncthreads();
-    float* part = tb.part + (size_t)blockIdx.x * (MLP_NP + 1);
-    for (int i = tid; i <= NSMALL; i += TRAIN_P) {
-        float s = 0.0f;
-        for (int w = 0; w < NW; ++w) s += wred[w][i];
+    // sums over points: thread r < 4H + 2 owns one bias / per-unit weight;
+    // four accumulators over interleaved points, combined in a fixed order
+    if (tid < 4 * H + 2) {
+        const uint32_t r = tid;
+        float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+        auto sum4 = [&](auto term) {
+#pragma unroll 4
+            for (int i = 0; i < TRAIN_P; i += 4) {
+                a0 += term(i);
+                a1 += term(i + 1);
+                a2 += term(i + 2);
+                a3 += term(i + 3);
+            }
+            return (a0 + a1) + (a2 + a3);
+        };
+        float sum;
         int dst;
-        if (i < H) dst = i;                                 // p1
-        else if (i < 2 * H) dst = MLP_OFF_B1 + (i - H);     // b1
-        else if (i < 3 * H) dst = MLP_OFF_B2 + (i - 2 * H); // b2
-        else if (i < 4 * H) dst = MLP_OFF_P3 + (i - 3 * H); // p3
-        else if (i == 4 * H) dst = MLP_OFF_B3;              // b3
-        else dst = MLP_NP;                                  // loss
-        part[dst] = s;
+        if (r < H) {                 // p1_j: 2 p1_j sum dd_j x
+            sum = 2.0f * p[r] * sum4([&](int i) { return L.sdd[r][i] * L.sx[i]; });
+            dst = r;
+        } else if (r < 2 * H) {      // b1_j: sum dd_j
+            sum = sum4([&](int i) { return L.sdd[r - H][i]; });
+            dst = MLP_OFF_B1 + (r - H);
+        } else if (r < 3 * H) {      // b2_k: p3_k^2 sum over active points of dy
+            const int k = r - 2 * H;
+            sum = sum4([&](int i) {
+                const float h = L.sh2[k][i];
+                return (h > 0.0f && h < 1.0f) ? L.sdy[i] : 0.0f;
+            });
+            sum *= p[MLP_OFF_P3 + k] * p[MLP_OFF_P3 + k];
+            dst = MLP_OFF_B2 + k;
+        } else if (r < 4 * H) {      // p3_k: 2 p3_k sum dy h2_k
+            const int k = r - 3 * H;
+            sum = 2.0f * p[MLP_OFF_P3 + k] * sum4([&](int i) { return L.sdy[i] * L.sh2[k][i]; });
+            dst = MLP_OFF_P3 + k;
+        } else if (r == 4 * H) {     // b3: sum dy
+            sum = sum4([&](int i) { return L.sdy[i]; });
+            dst = MLP_OFF_B3;
+        } else {                     // the workgroup's sum of squared errors
+            sum = sum4([&](int i) { return L.se2[i]; });
+            dst = MLP_NP;
+        }
+        out[dst] = sum;
     }
-    // layer-2 weights: G[k][j] = sum over points of d2_k * h1_j
-    for (int q = tid; q < H * H; q += TRAIN_P) {
-        const int k = q / H, j = q % H;
-        float s = 0.0f;
-        for (int i = 0; i < TRAIN_P; ++i) s = fmaf(sd2[k][i], sh1[j][i], s);
-        part[MLP_OFF_P2 + q] = 2.0f * p[MLP_OFF_P2 + q] * s;
+    // layer-2 weights: 2 p2_kj p3_k^2 sum over points active in unit k of dy h1_j,
+    // a 2x2 (k, j) tile per thread
+    {
+        const int k0 = 2 * (tid / (H / 2)), j0 = 2 * (tid % (H / 2));
+        float s00 = 0.0f, s01 = 0.0f, s10 = 0.0f, s11 = 0.0f;
+#pragma unroll 8
+        for (int i = 0; i < TRAIN_P; ++i) {
+            const float dy = L.sdy[i];
+            const float ha = L.sh2[k0][i], hb = L.sh2[k0 + 1][i];
+            const float da = (ha > 0.0f && ha < 1.0f) ? dy : 0.0f;
+            const float db = (hb > 0.0f && hb < 1.0f) ? dy : 0.0f;
+            const float xa = L.sh1[j0][i], xb = L.sh1[j0 + 1][i];
+            s00 = fmaf(da, xa, s00);
+            s01 = fmaf(da, xb, s01);
+            s10 = fmaf(db, xa, s10);
+            s11 = fmaf(db, xb, s11);
+        }
+        const float ca = 2.0f * p[MLP_OFF_P3 + k0] * p[MLP_OFF_P3 + k0];
+        const float cb = 2.0f * p[MLP_OFF_P3 + k0 + 1] * p[MLP_OFF_P3 + k0 + 1];
+        const int q = k0 * H + j0;
+        out[MLP_OFF_P2 + q] = ca * p[MLP_OFF_P2 + q] * s00;
+        out[MLP_OFF_P2 + q + 1] = ca * p[MLP_OFF_P2 + q + 1] * s01;
+        out[MLP_OFF_P2 + q + H] = cb * p[MLP_OFF_P2 + q + H] * s10;
+        out[MLP_OFF_P2 + q + H + 1] = cb * p[MLP_OFF_P2 + q + H + 1] * s11;
     }
+    __syncthreads();
+}
+
+static_assert(TRAIN_P == (MLP_H / 2) * (MLP_H / 2), "one 2x2 layer-2 weight tile per thread");
+static_assert(TRAIN_P >= 4 * MLP_H + 2, "one per-unit sum per thread");
+
+// One full-batch step over many workgroups: partial gradients per workgroup,
+// the last workgroup to finish sums them in workgroup order (deterministic,
+// no spin-waits) and updates the parameters -- or, grad_only, writes the
+// summed gradient to gout.
+__global__ __launch_bounds__(TRAIN_P) void k_mlp_step(TrainBufs tb, const float* u, const float* tg, uint32_t M,
+                                                      float lr, int grad_only, float* gout) {
+    __shared__ float p[MLP_NP];
+    __shared__ float w2[MLP_H * MLP_H];
+    __shared__ MlpLds L;
+    __shared__ uint32_t last;
+    const uint32_t tid = threadIdx.x;
+    for (int i = tid; i < MLP_NP; i += TRAIN_P) p[i] = tb.params[i];
+    __syncthreads();
+    for (int i = tid; i < MLP_H * MLP_H; i += TRAIN_P) w2[i] = p[MLP_OFF_P2 + i] * p[MLP_OFF_P2 + i];
+    __syncthreads();
+    const uint32_t pt = blockIdx.x * TRAIN_P + tid;
+    const bool valid = pt < M;
+    mlp_block_grad(p, w2, valid ? u[pt] : 0.0f, valid ? tg[pt] : 0.0f, valid, M, L,
+                   tb.part + (size_t)blockIdx.x * (MLP_NP + 1));
     __threadfence();
     __syncthreads();
     if (tid == 0) last = (atomicAdd(&tb.ctr[0], 1u) == gridDim.x - 1) ? 1u : 0u;
     __syncthreads();
     if (!last) return;
     __threadfence();
-    // last block: sum the partial gradients in block order
     const uint32_t step = tb.ctr[1] + 1;
     const float b1c = 1.0f / (1.0f - powf(0.9f, (float)step));
     const float b2c = 1.0f / (1.0f - powf(0.999f, (float)step));
@@ -203,76 +262,119 @@ __global__ __launch_bounds__(TRAIN_P) void k_mlp_step(TrainBufs tb, const float*
     }
 }
 
-// Number of sample keys whose feature is < x (sorted encoded sample).
-template <class K>
-__device__ __forceinline__ uint32_t rank_below(const K* smp, uint32_t ns, uint64_t kmin, float inv, float x) {
-    uint32_t lo = 0, hi = ns;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (learned_feature(smp[mid], kmin, inv) < x) lo = mid + 1;
-        else hi = mid;
+// Every training step in one workgroup (M <= TRAIN_P points, one per thread;
+// Adam with beta 0.9 / 0.999, bias-corrected, its moments in registers: thread
+// t owns parameters t, t + TRAIN_P, ...).  Writes the trained parameters and
+// the full-batch loss at them; with gout, also that loss's gradient (steps =
+// 0: the gradient at the given parameters).
+__global__ __launch_bounds__(TRAIN_P) void k_mlp_train(float* params, const float* u, const float* tg, uint32_t M,
+                                                       uint32_t steps, float lr, float* loss_out, float* gout) {
+    constexpr int OWN = (MLP_NP + TRAIN_P - 1) / TRAIN_P;
+    __shared__ float p[MLP_NP], g[MLP_NP + 1];
+    __shared__ float w2[MLP_H * MLP_H];
+    __shared__ MlpLds L;
+    const uint32_t tid = threadIdx.x;
+    float am[OWN], av[OWN];
+#pragma unroll
+    for (int q = 0; q < OWN; ++q) {
+        const uint32_t i = tid + q * TRAIN_P;
+        if (i < MLP_NP) p[i] = params[i];
+        am[q] = 0.0f;
+        av[q] = 0.0f;
     }
-    return lo;
+    const bool valid = tid < M;
+    const float x = valid ? u[tid] : 0.0f, target = valid ? tg[tid] : 0.0f;
+    float b1t = 1.0f, b2t = 1.0f;  // beta1^step, beta2^step
+    for (uint32_t step = 0;; ++step) {
+        __syncthreads();
+        for (int i = tid; i < MLP_H * MLP_H; i += TRAIN_P) w2[i] = p[MLP_OFF_P2 + i] * p[MLP_OFF_P2 + i];
+        __syncthreads();
+        mlp_block_grad(p, w2, x, target, valid, M, L, g);
+        if (step == steps) break;  // the loss (and gradient) at the final parameters
+        b1t *= 0.9f;
+        b2t *= 0.999f;
+        const float b1c = 1.0f / (1.0f - b1t), b2c = 1.0f / (1.0f - b2t);
+#pragma unroll
+        for (int q = 0; q < OWN; ++q) {
+            const uint32_t i = tid + q * TRAIN_P;
+            if (i >= MLP_NP) continue;
+            const float gi = g[i];
+            am[q] = 0.9f * am[q] + 0.1f * gi;
+            av[q] = 0.999f * av[q] + 0.001f * gi * gi;
+            p[i] -= lr * (am[q] * b1c) / (sqrtf(av[q] * b2c) + 1e-8f);
+        }
+    }
+    for (int i = tid; i < MLP_NP; i += TRAIN_P) {
+        params[i] = p[i];
+        if (gout) gout[i] = g[i];
+    }
+    if (tid == 0) *loss_out = g[MLP_NP] / (float)M;
 }
 
-// Empirical CDF of the sample at every grid point: ecdf[g] = rank / ns, and
-// the band half-width wgt[g] from the sample keys in the adjacent cells.
+// Point-mass check on the sorted sample: flag = 1 when some fp32 feature value
+// is shared by more than `run` consecutive sample keys.  The network sees a
+// key only through its feature and its CDF is continuous in it, so it cannot
+// place a bucket boundary inside such a value's keys.
 template <class K>
-__global__ __launch_bounds__(256) void k_calibrate(const K* smp, uint32_t ns, const uint64_t* geo, float* ecdf,
-                                                   float* wgt) {
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g > LEARN_G) return;
+__global__ __launch_bounds__(256) void k_point_mass(const K* smp, uint32_t ns, uint32_t run, const uint64_t* geo,
+                                                    uint32_t* flag) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i + run >= ns) return;
     const uint64_t kmin = geo[0], kmax = geo[1];
     const float inv = kmax > kmin ? 1.0f / (float)(kmax - kmin) : 0.0f;
-    const float x = (float)g / (float)LEARN_G;
-    const uint32_t r = g == 0 ? 0u : (g == LEARN_G ? ns : rank_below(smp, ns, kmin, inv, x));
-    const float xl = (float)(g > 0 ? g - 1 : 0) / (float)LEARN_G, xh = (float)(g < LEARN_G ? g + 1 : LEARN_G) / (float)LEARN_G;
-    const uint32_t rl = rank_below(smp, ns, kmin, inv, xl);
-    const uint32_t rh = g + 1 >= LEARN_G ? ns : rank_below(smp, ns, kmin, inv, xh);
-    ecdf[g] = (float)r / (float)ns;
-    // half-width of the band the network may use around the empirical CDF:
-    // half the sample mass of the two adjacent cells
-    wgt[g] = 0.5f * (float)(rh - rl) / (float)ns;
+    if (learned_feature(smp[i], kmin, inv) == learned_feature(smp[i + run], kmin, inv)) flag[0] = 1u;
 }
 
-// One block of 1024 threads: tab[g] = model(g / G) kept inside the sample's
-// band around its empirical CDF, then a running maximum so the table is
-// non-decreasing.
-__global__ __launch_bounds__(1024) void k_tabulate(const float* params, const float* ecdf, const float* wgt,
-                                                   float* tab) {
+__device__ __forceinline__ float mlp_eval(const float* p, float x) {
     constexpr int H = MLP_H;
-    constexpr int PER = (LEARN_G + 1 + 1023) / 1024;
+    float h1[H];
+#pragma unroll
+    for (int j = 0; j < H; ++j) h1[j] = ramp01(fmaf(p[j] * p[j], x, p[MLP_OFF_B1 + j]));
+    float y = p[MLP_OFF_B3];
+#pragma unroll 1
+    for (int k = 0; k < H; ++k) {
+        float z = p[MLP_OFF_B2 + k];
+#pragma unroll
+        for (int j = 0; j < H; ++j) z = fmaf(p[MLP_OFF_P2 + k * H + j] * p[MLP_OFF_P2 + k * H + j], h1[j], z);
+        y = fmaf(p[MLP_OFF_P3 + k] * p[MLP_OFF_P3 + k], ramp01(z), y);
+    }
+    return y;
+}
+
+// Knot k: the sample key of rank r_k = k (ns - 1) / LEARN_KNOTS and the
+// network's CDF there, kept within half the sample mass of the two adjacent
+// pieces around the empirical value (r_k + 0.5) / ns.
+template <class K>
+__global__ __launch_bounds__(256) void k_knots(const K* smp, uint32_t ns, const float* params, const uint64_t* geo,
+                                               uint64_t* key, float* cdf_raw) {
     __shared__ float p[MLP_NP];
+    for (int i = threadIdx.x; i < MLP_NP; i += blockDim.x) p[i] = params[i];
+    __syncthreads();
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > LEARN_KNOTS) return;
+    const uint64_t kmin = geo[0], kmax = geo[1];
+    const float inv = kmax > kmin ? 1.0f / (float)(kmax - kmin) : 0.0f;
+    auto rank = [&](uint32_t j) { return (uint32_t)(((uint64_t)j * (ns - 1)) / LEARN_KNOTS); };
+    const uint32_t r = rank(k);
+    const uint32_t rl = k > 0 ? rank(k - 1) : 0u, rh = k < LEARN_KNOTS ? rank(k + 1) : ns - 1;
+    const float emp = ((float)r + 0.5f) / (float)ns;
+    const float band = 0.5f * (float)(rh - rl) / (float)ns;
+    const float y = mlp_eval(p, learned_feature(smp[r], kmin, inv));
+    key[k] = (uint64_t)smp[r];
+    cdf_raw[k] = fminf(fmaxf(fminf(fmaxf(y, emp - band), emp + band), 0.0f), 1.0f);
+}
+
+// One workgroup of 1024 threads: cdf = running maximum of cdf_raw.
+__global__ __launch_bounds__(1024) void k_knot_scan(const float* cdf_raw, float* cdf) {
+    constexpr int PER = (LEARN_KNOTS + 1 + 1023) / 1024;
     __shared__ float wmax[16];
     const uint32_t tid = threadIdx.x;
-    for (int i = tid; i < MLP_NP; i += 1024) p[i] = params[i];
-    __syncthreads();
     float v[PER];
     float run = 0.0f;
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
-        const uint32_t g = tid * PER + q;
-        float c = 0.0f;
-        if (g <= LEARN_G) {
-            const float x = (float)g / (float)LEARN_G;
-            float h1[H];
-#pragma unroll
-            for (int j = 0; j < H; ++j) h1[j] = ramp01(fmaf(p[j] * p[j], x, p[MLP_OFF_B1 + j]));
-            float y = p[MLP_OFF_B3];
-            for (int k = 0; k < H; ++k) {
-                float z = p[MLP_OFF_B2 + k];
-#pragma unroll
-                for (int j = 0; j < H; ++j) z = fmaf(p[MLP_OFF_P2 + k * H + j] * p[MLP_OFF_P2 + k * H + j], h1[j], z);
-                y = fmaf(p[MLP_OFF_P3 + k] * p[MLP_OFF_P3 + k], ramp01(z), y);
-            }
-            y = fminf(fmaxf(y, 0.0f), 1.0f);
-            // the network may move a grid value only within the sample's
-            // resolution there (wgt = its half-width); where the sample has no
-            // key nearby (wgt = 0) the empirical value stands
-            c = fminf(fmaxf(y, ecdf[g] - wgt[g]), ecdf[g] + wgt[g]);
-            c = fminf(fmaxf(c, 0.0f), 1.0f);
-        }
-        run = fmaxf(run, c);
+        const uint32_t k = tid * PER + q;
+        run = fmaxf(run, k <= LEARN_KNOTS ? cdf_raw[k] : 0.0f);
         v[q] = run;
     }
     // inclusive max-scan of the per-thread maxima
@@ -290,8 +392,8 @@ __global__ __launch_bounds__(1024) void k_tabulate(const float* params, const fl
     const float excl = fmaxf(before, __shfl_up(m, 1) * (lane ? 1.0f : 0.0f));
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
-        const uint32_t g = tid * PER + q;
-        if (g <= LEARN_G) tab[g] = fmaxf(excl, v[q]);
+        const uint32_t k = tid * PER + q;
+        if (k <= LEARN_KNOTS) cdf[k] = fmaxf(excl, v[q]);
     }
 }
 
@@ -306,41 +408,35 @@ s512_status build_learned_model(const T* a, size_t n, hipStream_t st, DeviceStat
     while (f1 < leaves && f1 < (1ull << prm.max_logf)) f1 <<= 1;
     const uint32_t ns = (uint32_t)std::min<uint64_t>(
         n, std::min<uint64_t>(std::max<uint64_t>((n * prm.sample_permille) / 1000, 64 * f1), 1u << 26));
-    const uint32_t M = 1024, nblk = M / TRAIN_P;
+    const uint32_t M = TRAIN_P;
+    const int lf1 = std::min(std::max(1, leaves <= 1 ? 0 : 64 - __builtin_clzll(leaves - 1)), prm.max_logf);
+    // a feature value on more than `run` sample keys holds more than
+    // prm.imbalance times a first-level bucket's mean mass
+    const uint32_t run = (uint32_t)std::max<double>(1.0, std::ceil(prm.imbalance * (double)ns / (double)(1u << lf1)));
     SizeCounter sc;
     sc.take<K>(ns);
     sc.take<float>(M);
     sc.take<float>(M);
     sc.take<float>(MLP_NP);
-    sc.take<float>(MLP_NP);
-    sc.take<float>(MLP_NP);
-    sc.take<float>((size_t)nblk * (MLP_NP + 1));
-    sc.take<uint32_t>(2);
     sc.take<float>(1);
     sc.take<uint64_t>(2);
-    sc.take<float>(LEARN_G + 1);
-    sc.take<float>(LEARN_G + 1);
-    sc.take<float>(LEARN_G + 1);
+    sc.take<uint64_t>(LEARN_KNOTS + 1);
+    sc.take<float>(LEARN_KNOTS + 1);
+    sc.take<float>(LEARN_KNOTS + 1);
+    sc.take<uint32_t>(1);
     s512_status rc;
     if ((rc = ds.learn.reserve(sc.off, st))) return rc;
     Arena ar{(char*)ds.learn.ptr, 0};
     K* smp = ar.take<K>(ns);
     float* u = ar.take<float>(M);
     float* t = ar.take<float>(M);
-    TrainBufs tb;
-    tb.params = ar.take<float>(MLP_NP);
-    tb.am = ar.take<float>(MLP_NP);
-    tb.av = ar.take<float>(MLP_NP);
-    tb.part = ar.take<float>((size_t)nblk * (MLP_NP + 1));
-    tb.ctr = ar.take<uint32_t>(2);
-    tb.loss = ar.take<float>(1);
+    float* params = ar.take<float>(MLP_NP);
+    float* loss = ar.take<float>(1);
     uint64_t* geo = ar.take<uint64_t>(2);
-    float* tab = ar.take<float>(LEARN_G + 1);
-    float* ecdf = ar.take<float>(LEARN_G + 1);
-    float* wgt = ar.take<float>(LEARN_G + 1);
-    S512_HIP(hipMemsetAsync(tb.am, 0, MLP_NP * sizeof(float), st));
-    S512_HIP(hipMemsetAsync(tb.av, 0, MLP_NP * sizeof(float), st));
-    S512_HIP(hipMemsetAsync(tb.ctr, 0, 2 * sizeof(uint32_t), st));
+    uint64_t* key = ar.take<uint64_t>(LEARN_KNOTS + 1);
+    float* cdf_raw = ar.take<float>(LEARN_KNOTS + 1);
+    float* cdf = ar.take<float>(LEARN_KNOTS + 1);
+    uint32_t* flag = ar.take<uint32_t>(1);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (stats) {
         (void)hipEventCreate(&e0);
@@ -352,39 +448,43 @@ s512_status build_learned_model(const T* a, size_t n, hipStream_t st, DeviceStat
     // the sample is sorted by this library's own engine (lock already held)
     EngineParams sp = default_params(sizeof(K) == 8);
     if ((rc = sort_engine_locked<K, false>(smp, nullptr, ns, st, sp, nullptr, false, ds))) return rc;
-    k_learn_prep<K><<<1, 256, 0, st>>>(smp, ns, M, u, t, tb.params, geo);
+    k_learn_prep<K><<<1, 256, 0, st>>>(smp, ns, M, u, t, params, geo);
     S512_CHECK_LAUNCH();
-    for (uint32_t s = 0; s < prm.train_steps; ++s) {
-        k_mlp_step<<<nblk, TRAIN_P, 0, st>>>(tb, u, t, M, prm.learn_rate, 0, nullptr);
-        S512_CHECK_LAUNCH();
-    }
-    k_calibrate<K><<<(LEARN_G + 1 + 255) / 256, 256, 0, st>>>(smp, ns, geo, ecdf, wgt);
+    S512_HIP(hipMemsetAsync(flag, 0, sizeof(uint32_t), st));
+    k_point_mass<K><<<(ns + 255) / 256, 256, 0, st>>>(smp, ns, run, geo, flag);
     S512_CHECK_LAUNCH();
-    k_tabulate<<<1, 1024, 0, st>>>(tb.params, ecdf, wgt, tab);
+    k_mlp_train<<<1, TRAIN_P, 0, st>>>(params, u, t, M, prm.train_steps, prm.learn_rate, loss, nullptr);
+    S512_CHECK_LAUNCH();
+    k_knots<K><<<(LEARN_KNOTS + 1 + 255) / 256, 256, 0, st>>>(smp, ns, params, geo, key, cdf_raw);
+    S512_CHECK_LAUNCH();
+    k_knot_scan<<<1, 1024, 0, st>>>(cdf_raw, cdf);
     S512_CHECK_LAUNCH();
     if ((rc = ds.pinned2.reserve(4096, st))) return rc;
     char* h = (char*)ds.pinned2.ptr;
     S512_HIP(hipMemcpyAsync(h, geo, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-    S512_HIP(hipMemcpyAsync(h + 64, tb.loss, sizeof(float), hipMemcpyDeviceToHost, st));
+    S512_HIP(hipMemcpyAsync(h + 64, loss, sizeof(float), hipMemcpyDeviceToHost, st));
+    S512_HIP(hipMemcpyAsync(h + 128, flag, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     if (stats) (void)hipEventRecord(e1, st);
     S512_HIP(hipStreamSynchronize(st));
     uint64_t g2[2];
     memcpy(g2, h, sizeof(g2));
-    float loss;
-    memcpy(&loss, h + 64, sizeof(float));
-    out->table.tab = tab;
-    out->table.kmin = g2[0];
-    out->table.inv_range = g2[1] > g2[0] ? 1.0f / (float)(g2[1] - g2[0]) : 0.0f;
+    float lossv;
+    memcpy(&lossv, h + 64, sizeof(float));
+    out->knots = LearnedModelDev{key, cdf};
     out->degenerate = g2[1] <= g2[0];
-    out->loss = loss;
-    out->params = tb.params;
+    out->loss = lossv;
+    out->params = params;
+    uint32_t pm;
+    memcpy(&pm, h + 128, sizeof(pm));
+    out->point_mass = pm != 0;
+    out->lf1 = (uint32_t)lf1;
     if (stats) {
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, e0, e1);
         stats->kernel_ms[S512_PHASE_TRAIN] += ms;
-        stats->launches[S512_PHASE_TRAIN] += prm.train_steps;
+        stats->launches[S512_PHASE_TRAIN] += 1;
         stats->keys[S512_PHASE_TRAIN] += ns;
-        stats->model_loss = loss;
+        stats->model_loss = lossv;
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
     }
@@ -419,6 +519,15 @@ extern "C" s512_status s512_mlp_loss_grad_f32(const float* d_params, const float
     sc.take<uint32_t>(2);
     if ((rc = ds.arena.reserve(sc.off, st))) return rc;
     Arena ar{(char*)ds.arena.ptr, 0};
+    if (m <= TRAIN_P) {
+        // one workgroup: the training kernel itself, with zero steps
+        float* prm = ar.take<float>(MLP_NP);
+        S512_HIP(hipMemcpyAsync(prm, d_params, MLP_NP * sizeof(float), hipMemcpyDeviceToDevice, st));
+        k_mlp_train<<<1, TRAIN_P, 0, st>>>(prm, d_u, d_t, m, 0, 0.0f, d_loss, d_grad);
+        S512_CHECK_LAUNCH();
+        S512_HIP(hipStreamSynchronize(st));
+        return S512_OK;
+    }
     TrainBufs tb;
     tb.params = ar.take<float>(MLP_NP);
     tb.part = ar.take<float>((size_t)nblk * (MLP_NP + 1));

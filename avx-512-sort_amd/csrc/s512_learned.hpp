@@ -1,6 +1,6 @@
 // s512_learned.hpp -- learned partitioning: a monotone 1-32-32-1 fp32 MLP that
 // models the CDF of the keys, trained on the GPU on a sorted ~1% sample, then
-// tabulated into a monotone table that predicts bucket indices.  No
+// used to place the splitters of the first two partition levels.  No
 // counterpart in the reference.
 //
 // Model (same definition as the oracle's fp64 restatement):
@@ -9,21 +9,19 @@
 //        sample's smallest/largest encoded key.  Every step is monotone.
 //   h1 = r(p1^2 * u + b1), h2 = r(P2^2 h1 + b2), y = p3^2 . h2 + b3,
 //        r(z) = min(max(z,0),1).  Squared weights make y non-decreasing in u.
-// Prediction: the model is evaluated on a grid of G+1 points and each grid
-// value is calibrated against the sorted sample (second stage of a recursive
-// model index): c_g = clamp(m_g, e_g - d_g, e_g + d_g), e_g the sample's
-// empirical CDF at the grid point and d_g half the sample mass of the two
-// adjacent cells.  The network thus only smooths within the sample's local
-// resolution; where the sample has no key nearby the empirical value stands
-// (an unconstrained network value there, carried by the running maximum
-// below, would merge whole ranges into one bucket).  The table is
-// made non-decreasing by a running maximum, and a key's CDF value is the table
-// interpolated at u (clamped to the next knot, so the prediction is
-// non-decreasing in the key despite rounding).  The fine bucket is
-// fb = min(NB-1, floor(c * NB)); a partition level uses a bit field of fb.
-// Because fb is non-decreasing in the key, buckets are ordered ranges of keys
-// whatever the model's accuracy: the model only changes bucket sizes, never
-// the sorted result.
+// Knots: the sorted sample is cut into LEARN_KNOTS pieces of equal mass; knot
+// k is the sample key of rank r_k = k (ns-1) / LEARN_KNOTS and its CDF value
+// is the network's prediction at that key, kept within half the sample mass
+// of the two adjacent pieces around the empirical value (the network smooths
+// within the sample's resolution, it cannot move a knot further), then made
+// non-decreasing by a running maximum.  Equal-mass knots put the model's
+// resolution where the keys are, however skewed they are.
+// Splitters: the model's quantiles, i.e. the inverse of the piecewise-linear
+// CDF through the knots (k_model_splitters).  A level partitioned by the model
+// therefore uses the same classifier as sampled splitters; the model decides
+// where the bucket boundaries fall, and since its quantiles are non-decreasing
+// the buckets are ordered key ranges whatever its accuracy: the model only
+// changes bucket sizes, never the sorted result.
 #pragma once
 #include "s512_common.hpp"
 
@@ -36,17 +34,16 @@ constexpr int MLP_OFF_P2 = 2 * MLP_H;
 constexpr int MLP_OFF_B2 = 2 * MLP_H + MLP_H * MLP_H;
 constexpr int MLP_OFF_P3 = MLP_OFF_B2 + MLP_H;
 constexpr int MLP_OFF_B3 = MLP_OFF_P3 + MLP_H;
-constexpr int LEARN_G = 8192;       // table cells
 constexpr int TRAIN_P = 256;        // training workgroup (one point per thread)
 
 __device__ __forceinline__ float ramp01(float z) { return fminf(fmaxf(z, 0.0f), 1.0f); }
 
-// Tabulated model used by the count / scatter kernels.
-struct LearnedTable {
-    const float* tab;   // [LEARN_G + 1], non-decreasing, in [0,1]
-    uint64_t kmin;      // feature origin (encoded key)
-    float inv_range;    // 1 / (kmax - kmin)
-    uint32_t nb_log;    // log2 of the number of fine buckets
+// The model as the partition levels use it: LEARN_KNOTS + 1 knots (encoded
+// key, CDF value), both non-decreasing, in device memory.
+constexpr int LEARN_KNOTS = 8192;
+struct LearnedModelDev {
+    const uint64_t* key;
+    const float* cdf;
 };
 
 template <class K>
@@ -55,20 +52,21 @@ __device__ __forceinline__ float learned_feature(K x, uint64_t kmin, float inv_r
     return fminf((float)((uint64_t)x - kmin) * inv_range, 1.0f);
 }
 
-// Fine bucket of key x from the tabulated model held in LDS.
-template <class K>
-__device__ __forceinline__ uint32_t learned_fine_bucket(const float* tab, uint64_t kmin, float inv_range,
-                                                        uint32_t nb_log, K x) {
-    const float u = learned_feature(x, kmin, inv_range);
-    const float pos = u * (float)LEARN_G;
-    uint32_t gi = (uint32_t)pos;
-    gi = gi < LEARN_G - 1 ? gi : LEARN_G - 1;
-    const float fr = pos - (float)gi;
-    const float t0 = tab[gi], t1 = tab[gi + 1];
-    const float c = fminf(fmaf(t1 - t0, fr, t0), t1);
-    const float nb = (float)(1u << nb_log);
-    const uint32_t fb = (uint32_t)(c * nb);
-    return fb < (1u << nb_log) - 1 ? fb : (1u << nb_log) - 1;
+// Encoded key at CDF value q of the piecewise-linear model through the knots.
+__device__ __forceinline__ uint64_t model_quantile(const uint64_t* key, const float* cdf, double q) {
+    if (q <= (double)cdf[0]) return key[0];
+    if (q >= (double)cdf[LEARN_KNOTS]) return key[LEARN_KNOTS];
+    uint32_t lo = 1, hi = LEARN_KNOTS;  // first knot with cdf >= q, in [1, LEARN_KNOTS]
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if ((double)cdf[mid] < q) lo = mid + 1;
+        else hi = mid;
+    }
+    const double c0 = cdf[lo - 1], c1 = cdf[lo];
+    const double f = c1 > c0 ? (q - c0) / (c1 - c0) : 1.0;
+    const uint64_t k0 = key[lo - 1], k1 = key[lo];
+    const uint64_t d = (uint64_t)((double)(k1 - k0) * (f < 0.0 ? 0.0 : (f > 1.0 ? 1.0 : f)));
+    return k0 + (d > k1 - k0 ? k1 - k0 : d);
 }
 
 }  // namespace s512

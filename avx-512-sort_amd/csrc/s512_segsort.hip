@@ -281,7 +281,6 @@ static s512_status seg_enqueue(T* a, T* b, const uint64_t* off, size_t nseg, con
     S512_CHECK_LAUNCH();
     const uint32_t* nbig = w.counters + CNT_NBIG;
     const uint32_t* nchunk = w.counters + CNT_NCHUNK;
-    const LearnedTable no_model{nullptr, 0, 0.0f, 0};
     k_sample<T><<<grid_of(nbig_cap, 2048), 256, 0, st>>>(w.ps, nbig, a, b, (K*)w.samples, SEG_SMAX, prm.seed);
     S512_CHECK_LAUNCH();
     k_tile_sort<K, 512, 16><<<grid_of(nbig_cap, 2048), 512, 0, st>>>(w.sl, nbig, (K*)w.samples, (K*)w.samples);
@@ -290,14 +289,14 @@ static s512_status seg_enqueue(T* a, T* b, const uint64_t* off, size_t nseg, con
                                                               w.guides);
     S512_CHECK_LAUNCH();
     launch_count<T>(prm.count_shape, grid_of(nchunk_cap, 8192), st, w.ps, w.cr, nchunk, a, b, (K*)w.trees, w.guides,
-                    w.hist, w.counters + CNT_SPECIAL, no_model, nullptr);
+                    w.hist, w.counters + CNT_SPECIAL, nullptr);
     S512_CHECK_LAUNCH();
     k_colscan<<<grid_of((uint64_t)nbig_cap * CS_NSLICE, 8192), 256, 0, st>>>(w.ps, nbig, w.hist, w.bsize);
     S512_CHECK_LAUNCH();
     k_bucket_scan<<<grid_of(nbig_cap, 2048), BMAX, 0, st>>>(w.ps, nbig, w.bsize, w.bstart);
     S512_CHECK_LAUNCH();
     launch_scatter<T, false>(prm.scatter_shape, false, grid_of(nchunk_cap, 4096), st, w.ps, w.cr, a, b, nullptr,
-                             nullptr, (K*)w.trees, w.guides, w.hist, w.bstart, no_model, nchunk, nullptr);
+                             nullptr, (K*)w.trees, w.guides, w.hist, w.bstart, nchunk, nullptr);
     S512_CHECK_LAUNCH();
     k_plan<<<dim3(grid_of(nbig_cap, 2048), BMAX / 256), 256, 0, st>>>(w.ps, nbig, w.bstart, w.bsize, w.cls.pl);
     S512_CHECK_LAUNCH();
@@ -366,7 +365,7 @@ static s512_status seg_finish(T* a, T* b, const uint64_t* off, size_t nseg, size
     pf.ds = &ds;
     LevelState ls;
     return run_levels<T, false>(a, nullptr, b, nullptr, nkeys, std::move(segs), st, prm, nullptr, pf, ds,
-                                LearnedTable{nullptr, 0, 0.0f, 0}, ls);
+                                LearnedModelDev{nullptr, nullptr}, ls);
 }
 
 template <class T>

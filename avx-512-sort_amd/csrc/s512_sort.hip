@@ -17,6 +17,7 @@
 #include "s512_learned_host.hpp"
 #include "s512_engine_internal.hpp"
 #include <algorithm>
+#include <cmath>
 #include <type_traits>
 
 namespace s512 {
@@ -165,15 +166,14 @@ template void launch_tile_fixed<double>(int, int, const TileSeg*, const uint32_t
 template <class T, bool KV>
 s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vector<Seg> segs, hipStream_t st,
                        const EngineParams& prm, s512_stats* stats, Profiler& pf, DeviceState& ds,
-                       const LearnedTable& ltab, LevelState& lstate) {
+                       const LearnedModelDev& model, LevelState& lstate) {
     using K = typename KeyTraits<T>::K;
     using TC = TileCfg<T, KV>;
     s512_status rc;
     const int ncls = TC::nclass();
     bool& use_learned = lstate.use_learned;
-    const uint32_t lf1 = lstate.lf1, lf2 = lstate.lf2;
+    const uint32_t lf1 = lstate.lf1;
     bool learned_checked = false;
-    struct { LearnedTable table; } model{ltab};
     std::vector<PartSeg> ps;
     std::vector<ChunkRef> crs;
     const uint32_t smax = 8192;
@@ -211,22 +211,20 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
             p.lshift = 0;
             p.lbase = 0;
             if (use_learned && (segs[i].aux & 1u)) {
-                // learned: level 1 uses the top lf1 bits of the fine bucket, level 2 the low lf2 bits
+                // the model places the splitters: of the root level (quantile
+                // range [0, 1), 2^lf1 buckets), then of each root bucket j
+                // (range [j, j + 1) / 2^lf1, fan-out from the bucket's size)
                 p.mode = 1;
                 if (level == 1) {
                     lf = (int)lf1;
-                    p.lshift = lf2;
-                } else if (p.size > 2 * (n >> lf1)) {
-                    // the model under-split this range: sampled splitters below it
-                    p.mode = 0;
                 } else {
-                    lf = (int)std::max<uint32_t>(lf2, 1);
-                    p.lbase = (segs[i].aux >> 1) << lf2;
+                    p.lshift = lf1;
+                    p.lbase = segs[i].aux >> 1;
                 }
             }
             p.logf = (uint32_t)lf;
             const uint32_t F = 1u << lf, B = 2 * F;
-            p.nsample = std::min<uint32_t>(smax, prm.oversample * F);
+            p.nsample = p.mode ? F : std::min<uint32_t>(smax, prm.oversample * F);
             p.chunk = chunk;
             p.nchunks = (uint32_t)((p.size + chunk - 1) / chunk);
             p.hist_base = histN;
@@ -235,7 +233,6 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
             p.guide_base = sumG;
             p.kmin = 0;
             p.shift = 0;
-            if (p.mode) p.nsample = 0;
             p.flags = 0;
             p.sample_base = (uint64_t)i * smax;
             sumF += F;
@@ -319,22 +316,28 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
         bool any_split = false, any_learned = false;
         for (auto& p : ps) (p.mode ? any_learned : any_split) = true;
         if (any_split) {
-        pf.begin(S512_PHASE_SAMPLE);
-        k_sample<T><<<m, 256, 0, st>>>(d_ps, d_cnt + 12, a, b, d_smp, smax, seed);
-        S512_CHECK_LAUNCH();
-        pf.end();
-        pf.begin(S512_PHASE_SAMPLE_SORT);
-        k_tile_sort<K, 512, 16><<<nsplit, 512, 0, st>>>(d_sl, d_cnt + 10, d_smp, d_smp);
-        S512_CHECK_LAUNCH();
-        pf.end();
+            pf.begin(S512_PHASE_SAMPLE);
+            k_sample<T><<<m, 256, 0, st>>>(d_ps, d_cnt + 12, a, b, d_smp, smax, seed);
+            S512_CHECK_LAUNCH();
+            pf.end();
+            pf.begin(S512_PHASE_SAMPLE_SORT);
+            k_tile_sort<K, 512, 16><<<nsplit, 512, 0, st>>>(d_sl, d_cnt + 10, d_smp, d_smp);
+            S512_CHECK_LAUNCH();
+            pf.end();
+        }
+        if (any_learned) {
+            pf.begin(S512_PHASE_SAMPLE);
+            k_model_splitters<K><<<m, 256, 0, st>>>(d_ps, d_cnt + 12, model, d_smp);
+            S512_CHECK_LAUNCH();
+            pf.end();
+        }
         pf.begin(S512_PHASE_TREE);
         k_build_tree<K><<<m, FMAX, 0, st>>>(d_ps, d_cnt + 12, d_smp, smax, d_tree, d_guide);
         S512_CHECK_LAUNCH();
         pf.end();
-        }
         pf.begin(S512_PHASE_COUNT);
         launch_count<T>(prm.count_shape, tc, st, d_ps, d_cr, d_cnt + 13, a, b, d_tree, d_guide, d_hist, d_cnt + 11,
-                        model.table, d_ids);
+                        d_ids);
         S512_CHECK_LAUNCH();
         pf.end();
         pf.begin(S512_PHASE_SCAN);
@@ -344,8 +347,9 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
         S512_CHECK_LAUNCH();
         pf.end();
         if (any_learned && level == 1 && !learned_checked) {
-            // learned level 1: fall back to sampled splitters when the model
-            // leaves a bucket far above the mean (nothing has moved yet)
+            // learned root level: fall back to sampled splitters when the
+            // model leaves an ordinary bucket far above the mean (nothing has
+            // moved yet; equality buckets are finished and do not count)
             learned_checked = true;
             const uint32_t B1 = 2u << lf1;
             if ((rc = ds.pinned2.reserve(B1 * sizeof(uint64_t), st))) return rc;
@@ -353,7 +357,7 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
             S512_HIP(hipMemcpyAsync(hb, d_bsize, B1 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
             S512_HIP(hipStreamSynchronize(st));
             uint64_t mx = 0;
-            for (uint32_t j = 0; j < B1; ++j) mx = std::max(mx, hb[j]);
+            for (uint32_t j = 0; j < B1; j += 2) mx = std::max(mx, hb[j]);
             const double mean = (double)n / (double)(1u << lf1);
             if (stats) stats->learned_max_bucket_ratio = (float)((double)mx / mean);
             if ((double)mx > prm.imbalance * mean) {
@@ -363,11 +367,11 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
                 level = 0;
                 continue;
             }
-            if (stats) stats->learned_levels = lf2 ? 2 : 1;
+            if (stats) stats->learned_levels = 2;
         }
         pf.begin(S512_PHASE_SCATTER);
         launch_scatter<T, KV>(prm.scatter_shape, d_ids != nullptr, tc, st, d_ps, d_cr, a, b, va, vb, d_tree,
-                              d_guide, d_hist, d_bstart, model.table, d_cnt + 13, d_ids);
+                              d_guide, d_hist, d_bstart, d_cnt + 13, d_ids);
         S512_CHECK_LAUNCH();
         pf.end();
         pf.begin(S512_PHASE_PLAN);
@@ -490,16 +494,17 @@ s512_status sort_engine_locked(T* a, int32_t* va, size_t n, hipStream_t st, cons
     LearnedModel model;
     memset(&model, 0, sizeof(model));
     bool use_learned = false;
-    uint32_t lf1 = 0, lf2 = 0;
+    uint32_t lf1 = 0;
     if constexpr (!KV && (std::is_same<T, double>::value || std::is_same<T, int32_t>::value)) {
         if (prm.learned) {
             if ((rc = build_learned_model<T>(a, n, st, ds, prm, &model, stats))) return rc;
-            use_learned = !model.degenerate;
-            const uint64_t leaves = (n + prm.target_leaf - 1) / prm.target_leaf;
-            const int total = std::max(1, ceil_log2(leaves));
-            lf1 = (uint32_t)std::min(total, prm.max_logf);
-            lf2 = (uint32_t)std::max(0, std::min(total - (int)lf1, prm.max_logf));
-            model.table.nb_log = lf1 + lf2;
+            // rejected outright: a point mass the network cannot split, or a
+            // network that did not train (loss not finite or above the limit)
+            const bool fits = std::isfinite(model.loss) && model.loss <= prm.max_loss;
+            const bool ok = !model.degenerate && !model.point_mass && fits;
+            if (!model.degenerate && !ok && stats) stats->learned_fallback = 1;
+            use_learned = ok;
+            lf1 = model.lf1;
         }
     }
     // ---- scratch arrays (same size as the input)
@@ -512,16 +517,15 @@ s512_status sort_engine_locked(T* a, int32_t* va, size_t n, hipStream_t st, cons
     LevelState ls;
     ls.use_learned = use_learned;
     ls.lf1 = lf1;
-    ls.lf2 = lf2;
-    return run_levels<T, KV>(a, va, b, vb, n, segs, st, prm, stats, pf, ds, model.table, ls);
+    return run_levels<T, KV>(a, va, b, vb, n, segs, st, prm, stats, pf, ds, model.knots, ls);
 }
 
 template s512_status run_levels<int32_t, false>(int32_t*, int32_t*, int32_t*, int32_t*, size_t, std::vector<Seg>,
                                                 hipStream_t, const EngineParams&, s512_stats*, Profiler&, DeviceState&,
-                                                const LearnedTable&, LevelState&);
+                                                const LearnedModelDev&, LevelState&);
 template s512_status run_levels<double, false>(double*, int32_t*, double*, int32_t*, size_t, std::vector<Seg>,
                                                hipStream_t, const EngineParams&, s512_stats*, Profiler&, DeviceState&,
-                                               const LearnedTable&, LevelState&);
+                                               const LearnedModelDev&, LevelState&);
 template s512_status sort_engine<int32_t, false>(int32_t*, int32_t*, size_t, hipStream_t, const EngineParams&, s512_stats*, bool);
 template s512_status sort_engine_locked<uint64_t, false>(uint64_t*, int32_t*, size_t, hipStream_t, const EngineParams&, s512_stats*, bool, DeviceState&);
 template s512_status sort_engine_locked<uint32_t, false>(uint32_t*, int32_t*, size_t, hipStream_t, const EngineParams&, s512_stats*, bool, DeviceState&);
@@ -540,8 +544,9 @@ EngineParams default_params(bool wide) {
     p.f64_minmax = true;
     p.learned = false;
     p.sample_permille = 10;
-    p.train_steps = 100;
+    p.train_steps = 30;
     p.learn_rate = 2e-3f;
+    p.max_loss = 1e-3f;  // RMS CDF error ~3%: the network did not train
     p.imbalance = 4.0f;
     p.scatter_shape = 0;
     p.ids = true;

@@ -126,7 +126,7 @@ class Stats(ctypes.Structure):
 
 
 SYNTH = {"uniform": 0, "normal": 1, "distinct": 2, "sorted": 3, "reversed": 4, "index": 5, "mixture": 6,
-         "piecewise": 7}
+         "piecewise": 7, "zipf": 8}
 
 
 def fill(t, kind, seed=0, param=0):

@@ -62,6 +62,9 @@ def parse():
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 --pmc child runs")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-splitters comparison")
     ap.add_argument("--cpu-sample", type=int, default=1 << 25)
+    ap.add_argument("--workload", default="plain", choices=list(WORKLOADS),
+                    help="plain: the 256M fp64 headline (default); the others time the secondary workloads "
+                         "(key/value Zipf, segmented batches, learned 1G) and print their own JSON line")
     return ap.parse_args()
 
 
@@ -197,12 +200,159 @@ def learned_comparison(s512, n, reps=3):
     return out
 
 
+# ------------------------------------------------------------------ secondary workloads
+def _timed(run, restore, steps, warmup):
+    """Device ms of each timed run() (HIP events on torch's current stream, on
+    which the library also runs); restore() precedes every run, untimed."""
+    import torch
+    ms = []
+    for i in range(warmup + steps):
+        restore()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        run()
+        e1.record()
+        e1.synchronize()
+        if i >= warmup:
+            ms.append(e0.elapsed_time(e1))
+    return ms
+
+
+def _line(metric, unit, items, ms, steps, warmup, dtype, data, config, extra):
+    value = items / (sum(ms) / len(ms) / 1e3)
+    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
+           "ms_per_step": sum(ms) / len(ms), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": dtype, "data": data, "config": config}
+    out.update(extra)
+    return out
+
+
+def bench_kv_zipf(args):
+    """Sort512kv::Sort layout: 536,870,912 (int32 key, int32 value) pairs,
+    keys Zipf s=1.1 over 2^24 ranks through a seeded permutation, value =
+    original index.  Checked: keys non-decreasing, the pair multiset kept."""
+    import torch
+    import s512
+    n = args.n if args.n != 268_435_456 else 536_870_912
+    dev = torch.device("cuda:0")
+    k0 = torch.empty(n, dtype=torch.int32, device=dev)
+    s512.fill(k0, "zipf", seed=3, param=1 << 24)
+    v0 = torch.arange(n, dtype=torch.int32, device=dev)
+    k, v = torch.empty_like(k0), torch.empty_like(v0)
+    pair = lambda a, b: a.to(torch.int64) * (1 << 32) + b.to(torch.int64)  # noqa: E731
+    h0 = multiset_hash(pair(k0, v0))
+
+    def restore():
+        k.copy_(k0)
+        v.copy_(v0)
+    ms = _timed(lambda: s512.kv_sort_ex(k, v), restore, args.steps, args.warmup)
+    ok = is_sorted(k) and multiset_hash(pair(k, v)) == h0
+    distinct = int(torch.unique(k0).numel())
+    return _line("sorted key/value pairs/s, 512M int32 pairs, Zipf keys", "pairs/s", n, ms, args.steps,
+                 args.warmup, "int32", "synthetic (device-generated Zipf s=1.1 via rejection-inversion)",
+                 {"workload": f"kv sort, {n} (int32 key, int32 value) pairs, keys Zipf s=1.1 over 2^24 ranks "
+                              f"({distinct} distinct), value = index"},
+                 {"verified": ok, "effective_hbm_frac": (16.0 * n / (sum(ms) / len(ms) / 1e3) / 1e9) / HBM_PEAK_GBS})
+
+
+def _seg_hash(t, off):
+    import torch
+    x = t.view(torch.int64) if t.dtype == torch.float64 else t.to(torch.int64)
+    x = x * -7046029254386353131
+    x = x ^ (x >> 29)
+    c = torch.cat([torch.zeros(1, dtype=torch.int64, device=t.device), torch.cumsum(x, 0)])
+    return c[off[1:]] - c[off[:-1]]
+
+
+def _seg_sorted(t, off):
+    import torch
+    cont = torch.ones(t.numel(), dtype=torch.bool, device=t.device)
+    starts = off[:-1]
+    cont[starts[starts < t.numel()]] = False  # a segment's first key has no predecessor in it
+    return bool(torch.all((t[1:] >= t[:-1]) | ~cont[1:]).item())
+
+
+def bench_segsort(args, dtype):
+    """200,000 independent arrays in one captured launch (segmented sort plan),
+    sizes lognormal (median 512, sigma 1.5) clipped to [1, 2^20]; int32
+    uniform or fp64 normal keys.  Checked per segment: non-decreasing, same
+    multiset hash."""
+    import numpy as np
+    import torch
+    import s512
+    dev = torch.device("cuda:0")
+    off = s512.lognormal_offsets(200_000, seed=0)
+    n = int(off[-1])
+    toff = torch.from_numpy(off.view(np.int64)).to(dev)
+    tdt = torch.int32 if dtype == "int32" else torch.float64
+    k0 = torch.empty(n, dtype=tdt, device=dev)
+    s512.fill(k0, "uniform" if dtype == "int32" else "normal", seed=1)
+    k = torch.empty_like(k0)
+    h0 = _seg_hash(k0, toff)
+    plan = s512.SegsortPlan(k, toff)
+    ms = _timed(plan.launch, lambda: k.copy_(k0), args.steps, args.warmup)
+    ok = _seg_sorted(k, toff) and bool(torch.equal(_seg_hash(k, toff), h0))
+    plan.close()
+    kb = 4 if dtype == "int32" else 8
+    return _line(f"sorted keys/s, segmented batch of 200,000 {dtype} arrays", "keys/s", n, ms, args.steps,
+                 args.warmup, dtype, "synthetic (device-generated; sizes lognormal, numpy PCG64 seed 0)",
+                 {"workload": f"segmented sort plan (hipGraph), 200000 segments, {n} {dtype} keys, sizes "
+                              f"lognormal(median 512, sigma 1.5) clipped to [1, 2^20]"},
+                 {"verified": ok, "effective_hbm_frac": (2.0 * kb * n / (sum(ms) / len(ms) / 1e3) / 1e9) / HBM_PEAK_GBS})
+
+
+def bench_learned(args):
+    """1,073,741,824 fp64 keys: the Gaussian mixture + lognormal tail and the
+    piecewise-constant adversarial input, each sorted by the learned
+    partitioner and by sampled splitters (same input, both verified)."""
+    import torch
+    import s512
+    n = args.n if args.n != 268_435_456 else 1 << 30
+    dev = torch.device("cuda:0")
+    src = torch.empty(n, dtype=torch.float64, device=dev)
+    work = torch.empty_like(src)
+    res, ok = {}, True
+    for kind in ("mixture", "piecewise"):
+        s512.fill(src, kind, seed=7)
+        h0 = multiset_hash(src)
+        r = {}
+        for name in ("splitters", "learned"):
+            st = {}
+
+            def run():
+                st["s"] = s512.sort_learned(work) if name == "learned" else s512.sort_ex(work)
+            ms = _timed(run, lambda: work.copy_(src), args.steps, args.warmup)
+            ok = ok and is_sorted(work) and multiset_hash(work) == h0
+            r[name] = {"ms": sum(ms) / len(ms), "keys_per_s": n / (sum(ms) / len(ms) / 1e3)}
+            if name == "learned":
+                d = st["s"].as_dict()
+                r[name].update({"classifier": d["classifier"], "fallback": d["learned_fallback"],
+                                "max_bucket_ratio": round(d["learned_max_bucket_ratio"], 3),
+                                "model_loss": d["model_loss"]})
+        r["learned_vs_splitters"] = r["splitters"]["ms"] / r["learned"]["ms"]
+        res[kind] = r
+    ms = [res["mixture"]["learned"]["ms"]]
+    return _line("sorted keys/s, 1G fp64, learned partitioner", "keys/s", n, ms, args.steps, args.warmup, "f64",
+                 "synthetic (device-generated mixture / piecewise)",
+                 {"workload": f"learned-partition sort, {n} fp64 keys, 8-component Gaussian mixture + "
+                              f"lognormal(0,2) tail; piecewise-constant adversarial input alongside"},
+                 {"verified": ok, "learned": res})
+
+
+WORKLOADS = {"plain": None, "kv-zipf": bench_kv_zipf, "segsort-i32": lambda a: bench_segsort(a, "int32"),
+             "segsort-f64": lambda a: bench_segsort(a, "float64"), "learned-1g": bench_learned}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box in this setup has one GPU and the sort runs on one device "
               f"(asked for --gpus {args.gpus}).", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "plain":
+        out = WORKLOADS[args.workload](args)
+        print(json.dumps(out))
+        sys.exit(0 if out.get("verified") else 1)
     # counter runs first, before this process touches the GPU
     traffic_runs = None
     if not args.no_traffic:

@@ -108,7 +108,7 @@ s512_status s512_merge_inplace_f64_dev(double* d_a, size_t size, size_t center, 
 enum {
     S512_PHASE_SAMPLE = 0,      /* draw splitter samples                    */
     S512_PHASE_SAMPLE_SORT = 1, /* bitonic sort of the samples              */
-    S512_PHASE_TREE = 2,        /* splitter tree / learned model tabulation */
+    S512_PHASE_TREE = 2,        /* splitter tree (and model splitters)     */
     S512_PHASE_COUNT = 3,       /* classify + per-chunk bucket histogram    */
     S512_PHASE_SCAN = 4,        /* bucket offsets                           */
     S512_PHASE_SCATTER = 5,     /* classify + LDS-staged scatter            */
@@ -122,7 +122,9 @@ enum {
 enum {
     S512_CLASSIFY_SPLITTERS = 0, /* sampled splitters at every level (default) */
     S512_CLASSIFY_LEARNED = 1    /* learned CDF model for the first two levels,
-                                    sampled splitters as the fallback        */
+                                    sampled splitters as the fallback when the
+                                    model's training loss is too high or a
+                                    first-level bucket is too large          */
 };
 
 typedef struct s512_options {
@@ -133,7 +135,7 @@ typedef struct s512_options {
     uint64_t seed;          /* sample positions (0: fixed default)           */
     uint32_t classifier;    /* S512_CLASSIFY_*; learned applies to keys-only int32/double sorts */
     uint32_t sample_permille; /* learned: model sample, per mille of n (0: 10 = 1%) */
-    uint32_t train_steps;   /* learned: Adam steps (0: 100)                 */
+    uint32_t train_steps;   /* learned: Adam steps (0: 30)                  */
     uint32_t imbalance_pct; /* learned: fall back to splitters when a first-level bucket
                                exceeds this percentage of the mean (0: 400) */
     uint32_t scatter_variant; /* 0: default (1); else 1 + shape + 4 * reclassify, where shape

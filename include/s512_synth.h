@@ -19,8 +19,11 @@ enum {
     S512X_REVERSED = 4,  /* f64: (n-1-i)/n;  i32: n/2 - i                              */
     S512X_INDEX = 5,     /* i32 only: i                                                */
     S512X_MIXTURE = 6,   /* f64 only: 8-component Gaussian mixture + 10% lognormal(0,2) */
-    S512X_PIECEWISE = 7  /* f64 only: piecewise-constant density, param pieces of equal
+    S512X_PIECEWISE = 7, /* f64 only: piecewise-constant density, param pieces of equal
                             mass with widths alternating 1 and 1e-9 (default 64)     */
+    S512X_ZIPF = 8       /* i32 only: Zipf rank r in 1..N (P(r) ~ r^-1.1, N = param,
+                            default 2^24; rejection-inversion sampling), mapped to
+                            the key perm(r - 1) by a seeded bijection of [0, N)    */
 };
 
 s512_status s512x_fill_f64(double* d_out, size_t n, int kind, uint64_t seed, uint64_t param, void* stream);

@@ -5,7 +5,10 @@
 - A learned sort returns exactly the keys of the plain sort (bit-exact):
   the model only changes bucket sizes.
 - On the piecewise-constant adversarial input the model is rejected and the
-  sampled-splitter fallback runs (asserted through the stats).
+  sampled-splitter fallback runs (asserted through the stats): its 32 pieces
+  each put 1/64 of the keys on one fp32 feature value, a step the network's
+  continuous CDF cannot represent.
+- Training gains: the trained network's loss against the one-step network's.
 - The A/B test records the learned path's time against sampled splitters on
   the skewed mixture input.
 """
@@ -30,13 +33,15 @@ def _params(orc, H, rng):
     return p
 
 
+@pytest.mark.parametrize("m", [1000, 256, 100])
 @pytest.mark.parametrize("seed", [1, 2, 3])
-def test_training_gradient_matches_oracle(s512, orc, dev, seed):
+def test_training_gradient_matches_oracle(s512, orc, dev, seed, m):
+    """m <= 256 runs the single-workgroup training kernel with zero steps (the
+    kernel the learned sort trains with); m = 1000 the multi-workgroup step."""
     import torch
     H = 32
     rng = np.random.default_rng(seed)
     p = _params(orc, H, rng)
-    m = 1000
     u = np.sort(rng.uniform(0, 1, m))
     t = np.sort(rng.uniform(0, 1, m))
     loss_ref, g_ref = orc.mlp_loss_grad(p, H, u, t)
@@ -75,8 +80,9 @@ def test_learned_sort_exact(s512, dev, kind, n):
         # 32 pieces of width 1e-9 each hold 1/64 of the keys: 8x a first-level
         # bucket; the model must be rejected
         assert st.learned_fallback == 1 and st.classifier == 0, st.as_dict()
-    if kind == "uniform":
+    if kind in ("uniform", "normal", "mixture", "sorted"):
         assert st.learned_fallback == 0 and st.classifier == 1, st.as_dict()
+        assert st.learned_max_bucket_ratio < 2.0, st.as_dict()
     print(f"\n{kind} n={n}: classifier={st.classifier} fallback={st.learned_fallback} "
           f"max_bucket_ratio={st.learned_max_bucket_ratio:.2f} loss={st.model_loss:.3e}")
 
@@ -115,3 +121,25 @@ def test_learned_vs_splitters_ab(s512, dev, capsys, kind):
               f"learned {times['learned']*1e3:.2f} ms, ratio {times['splitters']/times['learned']:.3f}; "
               f"learned classifier={lst['classifier']} fallback={lst['learned_fallback']} "
               f"max_ratio={lst['learned_max_bucket_ratio']:.2f}")
+
+
+@pytest.mark.parametrize("kind", ["normal", "mixture"])
+def test_training_gain(s512, dev, capsys, kind):
+    """What training buys: the CDF fit (full-batch MSE of the training points)
+    of the trained network against the network after one Adam step from its
+    initialisation (the piecewise-linear interpolation through 33 sample
+    quantiles), and the first-level bucket balance each one gives."""
+    n = 16 << 20
+    src = _fill(s512, dev, kind, n, seed=5)
+    res = {}
+    for steps in (1, 30, 200):
+        t = src.clone()
+        st = s512.sort_learned(t, train_steps=steps, max_fanout_log2=9, target_leaf=1024)
+        assert st.classifier == 1, st.as_dict()
+        res[steps] = (st.model_loss, st.learned_max_bucket_ratio)
+    with capsys.disabled():
+        print(f"\n{kind} 16M: " + ", ".join(f"{k} steps: loss {v[0]:.2e} max/mean {v[1]:.3f}"
+                                           for k, v in res.items()))
+    assert res[30][0] < 0.5 * res[1][0], res
+    assert res[200][0] < res[30][0], res
+    assert res[30][1] <= res[1][1] * 1.05, res
