@@ -64,7 +64,8 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=1 << 25)
     ap.add_argument("--workload", default="plain", choices=list(WORKLOADS),
                     help="plain: the 256M fp64 headline (default); the others time the secondary workloads "
-                         "(key/value Zipf, segmented batches, learned 1G) and print their own JSON line")
+                         "(256M int32, key/value Zipf, segmented batches, learned 1G) and print their own "
+                         "JSON line")
     return ap.parse_args()
 
 
@@ -339,7 +340,50 @@ def bench_learned(args):
                  {"verified": ok, "learned": res})
 
 
-WORKLOADS = {"plain": None, "kv-zipf": bench_kv_zipf, "segsort-i32": lambda a: bench_segsort(a, "int32"),
+def bench_plain_i32(args):
+    """The plain sort on 268,435,456 int32 keys: full-range uniform, 1000
+    distinct values, sorted, reversed.  Roofline of the dominant kernel as in
+    the fp64 headline (8 B per key read + written per scatter launch)."""
+    import torch
+    import s512
+    n = args.n
+    dev = torch.device("cuda:0")
+    dists = ["uniform", "distinct", "sorted", "reversed"]
+    work = torch.empty(n, dtype=torch.int32, device=dev)
+    src = torch.empty_like(work)
+    per, ok = {}, True
+    ph_ms, ph_l, ph_k = {}, {}, {}
+    for i, d in enumerate(dists):
+        s512.fill(src, d, seed=i, param=1000)
+        h0 = multiset_hash(src)
+        st = {}
+
+        def run():
+            st["s"] = s512.sort_ex(work, profile=True)
+        ms = _timed(run, lambda: work.copy_(src), args.steps, args.warmup)
+        ok = ok and is_sorted(work) and multiset_hash(work) == h0
+        per[d] = sum(ms) / len(ms)
+        for j, name in enumerate(s512.PHASES):
+            ph_ms[name] = ph_ms.get(name, 0.0) + st["s"].kernel_ms[j]
+            ph_l[name] = ph_l.get(name, 0) + st["s"].launches[j]
+            ph_k[name] = ph_k.get(name, 0) + st["s"].keys[j]
+    bpk = {"count": 4, "scatter": 8, "tile": 8, "copy": 8}
+    dom = max((p for p in bpk if ph_l.get(p)), key=lambda p: ph_ms[p])
+    per_launch_bytes = ph_k[dom] * bpk[dom] / ph_l[dom]
+    per_launch_ms = ph_ms[dom] / ph_l[dom]
+    achieved = per_launch_bytes / (per_launch_ms / 1e3) / 1e9
+    total = sum(per.values())
+    return _line("sorted keys/s + % HBM peak, 256M int32", "keys/s", n * len(dists), [total], args.steps,
+                 args.warmup, "int32", "synthetic (device-generated SplitMix64; four distributions)",
+                 {"workload": f"plain sort, {n} int32 keys x {len(dists)} distributions ({','.join(dists)})"},
+                 {"verified": ok, "per_distribution_ms": per,
+                  "roofline": {"bound": "hbm", "kernel": KERNEL_OF_PHASE[dom], "achieved": achieved,
+                               "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                               "traffic": None, "bytes_per_launch": per_launch_bytes,
+                               "avg_launch_ms": per_launch_ms}})
+
+
+WORKLOADS = {"plain": None, "plain-i32": bench_plain_i32, "kv-zipf": bench_kv_zipf, "segsort-i32": lambda a: bench_segsort(a, "int32"),
              "segsort-f64": lambda a: bench_segsort(a, "float64"), "learned-1g": bench_learned}
 
 
