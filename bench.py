@@ -19,7 +19,8 @@ Output: one JSON line.
   traffic       HBM bytes per launch of that kernel, from two rocprofv3 --pmc
                 child runs of this script on the uniform input (FETCH_SIZE,
                 then WRITE_SIZE), each counter scaled by what it reads for the
-                run's own 2 GiB restore copy (a copy kernel of known size)
+                run's own 2 GiB restore copy (a copy kernel of known size);
+                measured_frac = traffic / launch time / peak
   learned       the learned partitioner against sampled splitters on the
                 skewed mixture and on uniform keys (same input, both exact)
   cpu_baseline  the reference's SortOmpPartition, compiled from the reference
@@ -483,6 +484,7 @@ def main():
                    "keys_per_step": n * len(dists), "classifier": "sampled splitters"},
         "roofline": {"bound": "hbm", "kernel": KERNEL_OF_PHASE[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "measured_frac": (traffic / (per_launch_ms / 1e3) / 1e9 / HBM_PEAK_GBS) if traffic else None,
                      "traffic_detail": traffic_detail, "bytes_per_launch": per_launch_bytes,
                      "avg_launch_ms": per_launch_ms},
         "effective_hbm_frac": (16.0 * keys / (total_ms / 1e3) / 1e9) / HBM_PEAK_GBS,
