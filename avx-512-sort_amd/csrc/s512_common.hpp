@@ -21,19 +21,25 @@ template <> struct KeyTraits<int32_t> {
     using K = uint32_t;
     static constexpr K KMAX = 0xFFFFFFFFu;
     __host__ __device__ static inline K enc(int32_t x) { return (uint32_t)x ^ 0x80000000u; }
+    __host__ __device__ static inline K bits(int32_t x) { return (uint32_t)x; }
+    __host__ __device__ static inline K enc_bits(K b) { return b ^ 0x80000000u; }
     __host__ __device__ static inline int32_t dec(K k) { return (int32_t)(k ^ 0x80000000u); }
 };
 
 template <> struct KeyTraits<double> {
     using K = uint64_t;
     static constexpr K KMAX = 0xFFFFFFFFFFFFFFFFull;
-    __host__ __device__ static inline K enc(double x) {
-        uint64_t b = __builtin_bit_cast(uint64_t, x);
-        return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+    // negative: all bits flipped; non-negative: sign bit set.  Branch-free:
+    // xor with (sign-extended sign bit | sign bit).
+    __host__ __device__ static inline K enc(double x) { return enc_bits(bits(x)); }
+    __host__ __device__ static inline K bits(double x) { return __builtin_bit_cast(uint64_t, x); }
+    __host__ __device__ static inline K enc_bits(K b) {
+        const uint64_t m = (uint64_t)((int64_t)b >> 63);
+        return b ^ (m | 0x8000000000000000ull);
     }
     __host__ __device__ static inline double dec(K k) {
-        uint64_t b = (k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
-        return __builtin_bit_cast(double, b);
+        const uint64_t m = ~(uint64_t)((int64_t)k >> 63);
+        return __builtin_bit_cast(double, k ^ (m | 0x8000000000000000ull));
     }
 };
 
@@ -43,6 +49,8 @@ template <class KT> struct RawKeyTraits {
     static constexpr K KMAX = (K)~(K)0;
     __host__ __device__ static inline K enc(K x) { return x; }
     __host__ __device__ static inline K dec(K k) { return k; }
+    __host__ __device__ static inline K bits(K x) { return x; }
+    __host__ __device__ static inline K enc_bits(K b) { return b; }
 };
 template <> struct KeyTraits<uint32_t> : RawKeyTraits<uint32_t> {};
 template <> struct KeyTraits<uint64_t> : RawKeyTraits<uint64_t> {};

@@ -675,16 +675,23 @@ __global__ __launch_bounds__(CNT_P) void k_count(const PartSeg* ps, const ChunkR
     const uint64_t lo = (uint64_t)c.idx * s.chunk;
     const uint64_t hi = lo + s.chunk < s.size ? lo + s.chunk : s.size;
     bool odd = false;
-    // clamped indices keep every load in bounds and unconditional
+    // clamped indices keep every load in bounds and unconditional; the buffers
+    // hold the raw bits, encoded only where a batch is classified, so that no
+    // instruction right after the prefetch has to wait for it
     auto load = [&](uint64_t base, K (&d)[CNT_U]) {
 #pragma unroll
         for (int u = 0; u < CNT_U; ++u) {
             const uint64_t i = base + (uint64_t)u * CNT_P + t;
-            d[u] = Tr::enc(src[i < hi ? i : hi - 1]);
+            d[u] = Tr::bits(src[i < hi ? i : hi - 1]);
         }
     };
-    auto batch = [&](uint64_t base, K (&x)[CNT_U], K (&xn)[CNT_U]) {
-        if (base + STEP < hi) load(base + STEP, xn);
+    auto batch = [&](uint64_t base, K (&xr)[CNT_U], K (&xn)[CNT_U]) {
+        // unconditional (clamped, in bounds past the end): a conditional
+        // prefetch makes the compiler wait for it before this batch's keys
+        load(base + STEP, xn);
+        K x[CNT_U];
+#pragma unroll
+        for (int u = 0; u < CNT_U; ++u) x[u] = Tr::enc_bits(xr[u]);
         if (base + STEP <= hi) {
             uint32_t bk[CNT_U];
 #pragma unroll
@@ -912,12 +919,15 @@ __global__ __launch_bounds__(P) void k_scatter(const PartSeg* ps, const ChunkRef
         int32_t v[E];
         uint32_t b[E];
     };
+    // a round at or past the end (the prefetch after the last round) reads
+    // key hi - 1 everywhere: in bounds, unconditional
     auto load_round = [&](uint64_t base, Buf& R) {
-        const uint32_t cnt = (uint32_t)((hi - base) < (uint64_t)ROUND ? (hi - base) : (uint64_t)ROUND);
+        const uint64_t bb = base < hi ? base : hi - 1;
+        const uint32_t cnt = (uint32_t)((hi - bb) < (uint64_t)ROUND ? (hi - bb) : (uint64_t)ROUND);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const uint32_t i = e * P + t;
-            const uint64_t ic = base + (i < cnt ? i : cnt - 1);  // in bounds, unconditional
+            const uint64_t ic = bb + (i < cnt ? i : cnt - 1);
             R.x[e] = src[ic];
             if constexpr (KV) R.v[e] = vsrc[ic];
             if constexpr (IDS) R.b[e] = isrc[ic];
@@ -925,7 +935,7 @@ __global__ __launch_bounds__(P) void k_scatter(const PartSeg* ps, const ChunkRef
     };
     auto do_round = [&](uint64_t base, Buf& cur, Buf& nxt) {
         const uint32_t cnt = (uint32_t)((hi - base) < (uint64_t)ROUND ? (hi - base) : (uint64_t)ROUND);
-        if (base + ROUND < hi) load_round(base + ROUND, nxt);  // prefetch
+        load_round(base + ROUND, nxt);  // prefetch, unconditional so that no wait covers it
         uint32_t bk[E], lr[E];
         if constexpr (IDS) {
 #pragma unroll

@@ -81,6 +81,7 @@ struct MlpLds {
     float sh1[MLP_H][TRAIN_P + 1];  // h1
     float sh2[MLP_H][TRAIN_P + 1];  // h2 (its ramp is active iff 0 < h2 < 1)
     float sdd[MLP_H][TRAIN_P + 1];  // layer-1 delta (ramp mask applied)
+    float sdm[MLP_H][TRAIN_P + 1];  // dy where unit k's layer-2 ramp is active, else 0
     float sdy[TRAIN_P];             // dLoss/dy
     float sx[TRAIN_P];              // feature
     float se2[TRAIN_P];             // squared error
@@ -129,7 +130,9 @@ __device__ __forceinline__ void mlp_block_grad(const float* p, const float* w2, 
     for (int k = 0; k < H; ++k) {
         const float p3 = p[MLP_OFF_P3 + k];
         const float h2 = L.sh2[k][tid];
-        const float d2 = (h2 > 0.0f && h2 < 1.0f) ? dy * p3 * p3 : 0.0f;
+        const float dm = (h2 > 0.0f && h2 < 1.0f) ? dy : 0.0f;
+        L.sdm[k][tid] = dm;
+        const float d2 = dm * p3 * p3;
 #pragma unroll
         for (int j = 0; j < H; ++j) d1[j] = fmaf(d2, w2[k * H + j], d1[j]);
     }
@@ -161,10 +164,7 @@ __device__ __forceinline__ void mlp_block_grad(const float* p, const float* w2, 
             dst = MLP_OFF_B1 + (r - H);
         } else if (r < 3 * H) {      // b2_k: p3_k^2 sum over active points of dy
             const int k = r - 2 * H;
-            sum = sum4([&](int i) {
-                const float h = L.sh2[k][i];
-                return (h > 0.0f && h < 1.0f) ? L.sdy[i] : 0.0f;
-            });
+            sum = sum4([&](int i) { return L.sdm[k][i]; });
             sum *= p[MLP_OFF_P3 + k] * p[MLP_OFF_P3 + k];
             dst = MLP_OFF_B2 + k;
         } else if (r < 4 * H) {      // p3_k: 2 p3_k sum dy h2_k
@@ -187,10 +187,7 @@ __device__ __forceinline__ void mlp_block_grad(const float* p, const float* w2, 
         float s00 = 0.0f, s01 = 0.0f, s10 = 0.0f, s11 = 0.0f;
 #pragma unroll 8
         for (int i = 0; i < TRAIN_P; ++i) {
-            const float dy = L.sdy[i];
-            const float ha = L.sh2[k0][i], hb = L.sh2[k0 + 1][i];
-            const float da = (ha > 0.0f && ha < 1.0f) ? dy : 0.0f;
-            const float db = (hb > 0.0f && hb < 1.0f) ? dy : 0.0f;
+            const float da = L.sdm[k0][i], db = L.sdm[k0 + 1][i];
             const float xa = L.sh1[j0][i], xb = L.sh1[j0 + 1][i];
             s00 = fmaf(da, xa, s00);
             s01 = fmaf(da, xb, s01);
