@@ -120,8 +120,39 @@ def cpu_baseline(sample_n):
         assert np.all(a[1:] >= a[:-1])
         best = dt if best is None else min(best, dt)
         runs += 1
-    return {"value": sample_n / best, "unit": "keys/s", "cores": cores, "kind": kind,
-            "sample": f"{what}, {sample_n} fp64 keys uniform [0,1) (SplitMix64 seed 0), best of {runs}"}
+    out = {"value": sample_n / best, "unit": "keys/s", "cores": cores, "kind": kind,
+           "sample": f"{what}, {sample_n} fp64 keys uniform [0,1) (SplitMix64 seed 0), best of {runs}",
+           "cpu": _cpu_model()}
+    if ref is not None:
+        # the reference's other entry points on the same input, one run each
+        # (Sort on one core on a 2^23-key prefix, to keep the run bounded)
+        extra = {}
+        fns = [("Sort512::Sort (1 core)", "ref_sort_f64", 1 << 23)]
+        if cores <= 128 and cores & (cores - 1) == 0:  # the merge variants need 2^k threads (sort512.hpp:6006)
+            fns += [("Sort512::SortOmpMerge", "ref_sort_omp_merge_f64", sample_n),
+                    ("Sort512::SortOmpMergeDeps", "ref_sort_omp_mergedeps_f64", sample_n),
+                    ("Sort512::SortOmpParMerge", "ref_sort_omp_parmerge_f64", sample_n)]
+        for name, fn_name, m in fns:
+            a = a0[:m].copy()
+            t0 = time.perf_counter()
+            getattr(ref, fn_name)(ctypes.c_void_p(a.ctypes.data), a.size)
+            dt = time.perf_counter() - t0
+            if not np.all(a[1:] >= a[:-1]):
+                raise RuntimeError(f"{name} did not sort")
+            extra[name] = {"keys": m, "keys_per_s": m / dt}
+        out["other_reference_functions"] = extra
+    return out
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 # ------------------------------------------------------------------ traffic

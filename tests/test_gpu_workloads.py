@@ -6,6 +6,8 @@
 - Key/value sort (Sort512kv::Sort, sort512kv.hpp) of Zipf keys: exact against
   numpy at 8M pairs; at 536,870,912 pairs by size-independent properties
   (keys non-decreasing, the (key, value) multiset kept, values a permutation).
+- Plain sorts at 268,435,456 keys (fp64 five distributions, int32 four):
+  non-decreasing, multiset kept.
 - Learned partitioner at 1,073,741,824 fp64 keys: mixture accepted, the
   piecewise-constant input rejected (fallback), both outputs sorted
   permutations of the input.
@@ -106,3 +108,31 @@ def test_learned_1g(s512, dev, kind):
         assert st.learned_fallback == 1 and st.classifier == 0, st.as_dict()
     else:
         assert st.learned_fallback == 0 and st.classifier == 1, st.as_dict()
+
+
+@pytest.mark.parametrize("kind", ["uniform", "normal", "distinct", "sorted", "reversed"])
+def test_plain_f64_full_size(s512, dev, kind):
+    """268,435,456 fp64 keys (the benchmark workload): non-decreasing and the
+    same multiset of bit patterns as the input."""
+    import torch
+    n = 268_435_456
+    t = torch.empty(n, dtype=torch.float64, device=dev)
+    s512.fill(t, kind, seed=0, param=1000)
+    h0 = _hash64(t.view(torch.int64))
+    st = s512.sort_ex(t)
+    assert st.levels >= 1
+    assert bool(torch.all(t[1:] >= t[:-1]).item())
+    assert _hash64(t.view(torch.int64)) == h0
+
+
+@pytest.mark.parametrize("kind", ["uniform", "distinct", "sorted", "reversed"])
+def test_plain_i32_full_size(s512, dev, kind):
+    """268,435,456 int32 keys over the full range."""
+    import torch
+    n = 268_435_456
+    t = torch.empty(n, dtype=torch.int32, device=dev)
+    s512.fill(t, kind, seed=0, param=1000)
+    h0 = _hash64(t.to(torch.int64))
+    s512.sort(t)
+    assert bool(torch.all(t[1:] >= t[:-1]).item())
+    assert _hash64(t.to(torch.int64)) == h0
