@@ -677,12 +677,14 @@ __global__ __launch_bounds__(CNT_P) void k_count(const PartSeg* ps, const ChunkR
     bool odd = false;
     // clamped indices keep every load in bounds and unconditional; the buffers
     // hold the raw bits, encoded only where a batch is classified, so that no
-    // instruction right after the prefetch has to wait for it
+    // instruction right after the prefetch has to wait for it.  Non-temporal:
+    // the keys are read once here (measured 4% faster); the scatter's loads
+    // stay ordinary, where L2 residency measured faster.
     auto load = [&](uint64_t base, K (&d)[CNT_U]) {
 #pragma unroll
         for (int u = 0; u < CNT_U; ++u) {
             const uint64_t i = base + (uint64_t)u * CNT_P + t;
-            d[u] = Tr::bits(src[i < hi ? i : hi - 1]);
+            d[u] = Tr::bits(__builtin_nontemporal_load(&src[i < hi ? i : hi - 1]));
         }
     };
     auto batch = [&](uint64_t base, K (&xr)[CNT_U], K (&xn)[CNT_U]) {

@@ -15,6 +15,7 @@
 // compared in the encoded order (see s512_common.hpp).
 #include "s512_kernels.hpp"
 #include "s512_runtime.hpp"
+#include "s512_hostio.hpp"
 #include "../../include/s512.h"
 #include <algorithm>
 #include <vector>
@@ -188,18 +189,7 @@ static s512_status merge_sort_dev(T* a, size_t n, hipStream_t st) {
 
 template <class T, class Fn>
 static s512_status on_device(T* a, size_t n, Fn fn) {
-    if (is_device_pointer(a)) return fn(a, (hipStream_t)0);
-    T* d = nullptr;
-    S512_HIP(hipMalloc(&d, n * sizeof(T)));
-    hipError_t e = hipMemcpy(d, a, n * sizeof(T), hipMemcpyHostToDevice);
-    s512_status rc = S512_OK;
-    if (e == hipSuccess) {
-        rc = fn(d, (hipStream_t)0);
-        if (rc == S512_OK) e = hipMemcpy(a, d, n * sizeof(T), hipMemcpyDeviceToHost);
-    }
-    (void)hipFree(d);
-    if (e != hipSuccess) { set_error("host staging copy failed: %s", hipGetErrorString(e)); return S512_ERR_HIP; }
-    return rc;
+    return with_staged_array(a, n, fn);
 }
 
 }  // namespace s512

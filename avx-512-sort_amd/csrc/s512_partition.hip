@@ -18,6 +18,7 @@
 // array.  Keys are read twice and written twice.
 #include "s512_common.hpp"
 #include "s512_runtime.hpp"
+#include "s512_hostio.hpp"
 #include "../../include/s512.h"
 
 namespace s512 {
@@ -183,22 +184,10 @@ static s512_status partition_dev(T* a, int32_t* va, size_t left, size_t right, T
     return S512_OK;
 }
 
+// host arrays: stage [0, right] (the reference indexes from the base pointer)
 template <class T, class Fn>
 static s512_status on_device_range(T* a, size_t right, Fn fn) {
-    // host arrays: stage [0, right] (the reference indexes from the base pointer)
-    if (is_device_pointer(a)) return fn(a, (hipStream_t)0);
-    const size_t n = right + 1;
-    T* d = nullptr;
-    S512_HIP(hipMalloc(&d, n * sizeof(T)));
-    hipError_t e = hipMemcpy(d, a, n * sizeof(T), hipMemcpyHostToDevice);
-    s512_status rc = S512_OK;
-    if (e == hipSuccess) {
-        rc = fn(d, (hipStream_t)0);
-        if (rc == S512_OK) e = hipMemcpy(a, d, n * sizeof(T), hipMemcpyDeviceToHost);
-    }
-    (void)hipFree(d);
-    if (e != hipSuccess) { set_error("host staging copy failed: %s", hipGetErrorString(e)); return S512_ERR_HIP; }
-    return rc;
+    return with_staged_array(a, right + 1, fn);
 }
 
 }  // namespace s512
@@ -238,9 +227,7 @@ extern "C" s512_status s512kv_partition_i32(int32_t* k, int32_t* v, size_t left,
     if (!out_bound) { set_error("null out_bound"); return S512_ERR_INVALID; }
     if (right + 1 <= left) { *out_bound = left; return S512_OK; }
     if (!k || !v) { set_error("null array"); return S512_ERR_INVALID; }
-    return on_device_range(k, right, [&](int32_t* dk, hipStream_t) {
-        return on_device_range(v, right, [&](int32_t* dv, hipStream_t st) {
-            return s512kv_partition_i32_dev(dk, dv, left, right, pivot, out_bound, st);
-        });
+    return with_staged_pair(k, v, right + 1, [&](int32_t* dk, int32_t* dv, hipStream_t st) {
+        return s512kv_partition_i32_dev(dk, dv, left, right, pivot, out_bound, st);
     });
 }

@@ -22,6 +22,7 @@
 // A bucket still larger than the largest tile lands on the "next" list; the
 // host finishes those with the multi-level engine after the sequence.
 #include "s512_engine_internal.hpp"
+#include "s512_hostio.hpp"
 #include "../../include/s512.h"
 #include <algorithm>
 #include <string.h>
@@ -463,31 +464,20 @@ static s512_status plan_launch(SegPlan* p, hipStream_t st, s512_stats* stats) {
     return seg_finish<T>((T*)p->keys, (T*)p->scratch, p->off, p->nseg, p->nkeys, p->w, st, prm, ds, stats);
 }
 
-// Host arrays: stage keys and offsets on the device.
+// Host arrays: stage keys and offsets on the device (pipelined copies).
 template <class T>
 static s512_status segsort_host(T* keys, const uint64_t* off, size_t nseg) {
     if (nseg == 0) return S512_OK;
     if (!keys || !off) { set_error("null pointer"); return S512_ERR_INVALID; }
     const size_t nkeys = off[nseg];
     if (nkeys < off[0]) { set_error("offsets decrease"); return S512_ERR_INVALID; }
-    T* dk = nullptr;
-    uint64_t* doff = nullptr;
-    S512_HIP(hipMalloc(&dk, std::max<size_t>(nkeys, 1) * sizeof(T)));
-    hipError_t e = hipMalloc(&doff, (nseg + 1) * sizeof(uint64_t));
-    s512_status rc = S512_OK;
-    if (e == hipSuccess) e = hipMemcpy(dk, keys, nkeys * sizeof(T), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(doff, off, (nseg + 1) * sizeof(uint64_t), hipMemcpyHostToDevice);
-    if (e == hipSuccess) {
-        rc = segsort_dev<T>(dk, doff, nseg, (hipStream_t)0, nullptr);
-        if (rc == S512_OK) e = hipMemcpy(keys, dk, nkeys * sizeof(T), hipMemcpyDeviceToHost);
-    }
-    (void)hipFree(dk);
-    if (doff) (void)hipFree(doff);
-    if (e != hipSuccess) {
-        set_error("host staging failed: %s", hipGetErrorString(e));
-        return S512_ERR_HIP;
-    }
-    return rc;
+    HostStage hs;
+    void* dk = nullptr;
+    void* doff = nullptr;
+    if (s512_status rc = hs.in(0, keys, nkeys * sizeof(T), &dk)) return rc;
+    if (s512_status rc = hs.in(2, off, (nseg + 1) * sizeof(uint64_t), &doff)) return rc;
+    if (s512_status rc = segsort_dev<T>((T*)dk, (const uint64_t*)doff, nseg, (hipStream_t)0, nullptr)) return rc;
+    return hs.out(0, keys, nkeys * sizeof(T));
 }
 
 }  // namespace s512

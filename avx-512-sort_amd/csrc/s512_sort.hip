@@ -16,6 +16,7 @@
 #include "s512_tile_f64.hpp"
 #include "s512_learned_host.hpp"
 #include "s512_engine_internal.hpp"
+#include "s512_hostio.hpp"
 #include <algorithm>
 #include <cmath>
 #include <type_traits>
@@ -620,36 +621,6 @@ EngineParams params_for(const s512_options* opt, int kind, uint64_t n) {
     return p;
 }
 
-// ---------------------------------------------------------------- host pointers
-// Run `fn` on device memory: in place for device pointers, through a staging
-// copy for host pointers.
-template <class T, class Fn>
-static s512_status with_device_array(T* a, size_t n, Fn fn) {
-    if (!a) {
-        set_error("null array pointer");
-        return S512_ERR_INVALID;
-    }
-    if (is_device_pointer(a)) {
-        return fn(a, (hipStream_t)0);
-    }
-    int dev;
-    s512_status rc = current_device(&dev);
-    if (rc) return rc;
-    T* d = nullptr;
-    S512_HIP(hipMalloc(&d, n * sizeof(T)));
-    hipError_t e = hipMemcpy(d, a, n * sizeof(T), hipMemcpyHostToDevice);
-    if (e == hipSuccess) {
-        rc = fn(d, (hipStream_t)0);
-        if (rc == S512_OK) e = hipMemcpy(a, d, n * sizeof(T), hipMemcpyDeviceToHost);
-    }
-    (void)hipFree(d);
-    if (e != hipSuccess) {
-        set_error("host staging copy failed: %s", hipGetErrorString(e));
-        return S512_ERR_HIP;
-    }
-    return rc;
-}
-
 }  // namespace s512
 
 using namespace s512;
@@ -715,11 +686,11 @@ extern "C" s512_status s512_sort_f64_dev(double* d_a, size_t n, void* stream) {
 }
 extern "C" s512_status s512_sort_i32(int32_t* a, size_t n) {
     if (n <= 1) return S512_OK;
-    return with_device_array(a, n, [n](int32_t* d, hipStream_t st) { return s512_sort_i32_dev(d, n, st); });
+    return with_staged_array(a, n, [n](int32_t* d, hipStream_t st) { return s512_sort_i32_dev(d, n, st); });
 }
 extern "C" s512_status s512_sort_f64(double* a, size_t n) {
     if (n <= 1) return S512_OK;
-    return with_device_array(a, n, [n](double* d, hipStream_t st) { return s512_sort_f64_dev(d, n, st); });
+    return with_staged_array(a, n, [n](double* d, hipStream_t st) { return s512_sort_f64_dev(d, n, st); });
 }
 extern "C" s512_status s512kv_sort_i32_dev(int32_t* d_k, int32_t* d_v, size_t n, void* stream) {
     return s512kv_sort_i32_dev_ex(d_k, d_v, n, nullptr, nullptr, stream);
@@ -727,8 +698,8 @@ extern "C" s512_status s512kv_sort_i32_dev(int32_t* d_k, int32_t* d_v, size_t n,
 extern "C" s512_status s512kv_sort_i32(int32_t* k, int32_t* v, size_t n) {
     if (n <= 1) return S512_OK;
     if (!k || !v) { set_error("null array"); return S512_ERR_INVALID; }
-    return with_device_array(k, n, [&](int32_t* dk, hipStream_t st) {
-        return with_device_array(v, n, [&](int32_t* dv, hipStream_t st2) { return s512kv_sort_i32_dev(dk, dv, n, st2); });
+    return with_staged_pair(k, v, n, [n](int32_t* dk, int32_t* dv, hipStream_t st) {
+        return s512kv_sort_i32_dev(dk, dv, n, st);
     });
 }
 

@@ -65,8 +65,8 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=1 << 25)
     ap.add_argument("--workload", default="plain", choices=list(WORKLOADS),
                     help="plain: the 256M fp64 headline (default); the others time the secondary workloads "
-                         "(256M int32, key/value Zipf, segmented batches, learned 1G) and print their own "
-                         "JSON line")
+                         "(256M int32, host-pointer end to end, key/value Zipf, segmented batches, learned 1G) "
+                         "and print their own JSON line")
     return ap.parse_args()
 
 
@@ -415,7 +415,47 @@ def bench_plain_i32(args):
                                "avg_launch_ms": per_launch_ms}})
 
 
-WORKLOADS = {"plain": None, "plain-i32": bench_plain_i32, "kv-zipf": bench_kv_zipf, "segsort-i32": lambda a: bench_segsort(a, "int32"),
+def bench_host(args):
+    """Sort512::Sort on a host array (the reference's own calling form):
+    268,435,456 fp64 keys in ordinary pageable host memory, end to end (host
+    to device, sort, device to host, through the pipelined staging), against
+    the PCIe bound measured in the same run: a pinned-memory copy of the
+    array to the device and back."""
+    import numpy as np
+    import torch
+    import s512
+    n = args.n
+    a0 = np.random.default_rng(0).random(n)
+    pin = torch.empty(n, dtype=torch.float64, pin_memory=True)
+    dev = torch.empty(n, dtype=torch.float64, device="cuda:0")
+    bound = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dev.copy_(pin)
+        pin.copy_(dev)
+        torch.cuda.synchronize()
+        bound.append(time.perf_counter() - t0)
+    del pin, dev
+    torch.cuda.empty_cache()
+    ms, ok = [], True
+    for i in range(args.warmup + args.steps):
+        a = a0.copy()
+        t0 = time.perf_counter()
+        s512.sort(a)
+        dt = time.perf_counter() - t0
+        if i >= args.warmup:
+            ms.append(dt * 1e3)
+        ok = ok and bool(np.all(a[1:] >= a[:-1]))
+    pcie_ms = min(bound) * 1e3
+    t = sum(ms) / len(ms)
+    return _line("sorted keys/s end to end from host memory, 256M fp64", "keys/s", n, [t], args.steps, args.warmup,
+                 "f64", "synthetic (numpy PCG64 uniform, pageable host memory)",
+                 {"workload": f"host-pointer sort, {n} fp64 keys, pageable host array"},
+                 {"verified": ok, "pcie_bound_ms": pcie_ms, "ratio_to_pcie_bound": t / pcie_ms})
+
+
+WORKLOADS = {"plain": None, "plain-i32": bench_plain_i32, "host": bench_host, "kv-zipf": bench_kv_zipf, "segsort-i32": lambda a: bench_segsort(a, "int32"),
              "segsort-f64": lambda a: bench_segsort(a, "float64"), "learned-1g": bench_learned}
 
 
