@@ -43,7 +43,17 @@ template <> struct KeyTraits<double> {
     }
 };
 
-// Identity traits: arrays that already hold encoded keys (splitter samples).
+template <> struct KeyTraits<int64_t> {
+    using K = uint64_t;
+    static constexpr K KMAX = 0xFFFFFFFFFFFFFFFFull;
+    __host__ __device__ static inline K enc(int64_t x) { return (uint64_t)x ^ 0x8000000000000000ull; }
+    __host__ __device__ static inline int64_t dec(K k) { return (int64_t)(k ^ 0x8000000000000000ull); }
+    __host__ __device__ static inline K bits(int64_t x) { return (uint64_t)x; }
+    __host__ __device__ static inline K enc_bits(K b) { return b ^ 0x8000000000000000ull; }
+};
+
+// Identity traits: unsigned keys, and arrays that already hold encoded keys
+// (splitter samples).
 template <class KT> struct RawKeyTraits {
     using K = KT;
     static constexpr K KMAX = (K)~(K)0;

@@ -532,6 +532,9 @@ template s512_status sort_engine_locked<uint64_t, false>(uint64_t*, int32_t*, si
 template s512_status sort_engine_locked<uint32_t, false>(uint32_t*, int32_t*, size_t, hipStream_t, const EngineParams&, s512_stats*, bool, DeviceState&);
 template s512_status sort_engine<double, false>(double*, int32_t*, size_t, hipStream_t, const EngineParams&, s512_stats*, bool);
 template s512_status sort_engine<int32_t, true>(int32_t*, int32_t*, size_t, hipStream_t, const EngineParams&, s512_stats*, bool);
+template s512_status sort_engine<int64_t, false>(int64_t*, int32_t*, size_t, hipStream_t, const EngineParams&, s512_stats*, bool);
+template s512_status sort_engine<uint64_t, false>(uint64_t*, int32_t*, size_t, hipStream_t, const EngineParams&, s512_stats*, bool);
+template s512_status sort_engine<uint32_t, false>(uint32_t*, int32_t*, size_t, hipStream_t, const EngineParams&, s512_stats*, bool);
 
 EngineParams default_params(bool wide) {
     EngineParams p;
@@ -692,6 +695,32 @@ extern "C" s512_status s512_sort_f64(double* a, size_t n) {
     if (n <= 1) return S512_OK;
     return with_staged_array(a, n, [n](double* d, hipStream_t st) { return s512_sort_f64_dev(d, n, st); });
 }
+// 64-bit and unsigned keys (no reference counterpart): the same engine, tuned
+// as the key width's reference type.
+#define S512_WIDE_KEYS(NAME, T, KIND)                                                                         \
+    extern "C" s512_status s512_sort_##NAME##_dev_ex(T* d_a, size_t n, const s512_options* opt, s512_stats* stats, \
+                                                     void* stream) {                                          \
+        if (n > 1 && !d_a) { set_error("null array"); return S512_ERR_INVALID; }                             \
+        if (s512_status rc = check_opts(opt)) return rc;                                                      \
+        if (opt && opt->classifier == S512_CLASSIFY_LEARNED) {                                                \
+            set_error("the learned classifier covers int32 and double keys");                                 \
+            return S512_ERR_INVALID;                                                                          \
+        }                                                                                                     \
+        return sort_engine<T, false>(d_a, nullptr, n, (hipStream_t)stream, params_for(opt, KIND, n), stats,   \
+                                     opt && opt->profile);                                                    \
+    }                                                                                                         \
+    extern "C" s512_status s512_sort_##NAME##_dev(T* d_a, size_t n, void* stream) {                          \
+        return s512_sort_##NAME##_dev_ex(d_a, n, nullptr, nullptr, stream);                                   \
+    }                                                                                                         \
+    extern "C" s512_status s512_sort_##NAME(T* a, size_t n) {                                                \
+        if (n <= 1) return S512_OK;                                                                           \
+        return with_staged_array(a, n, [n](T* d, hipStream_t st) { return s512_sort_##NAME##_dev(d, n, st); }); \
+    }
+S512_WIDE_KEYS(i64, int64_t, TUNE_F64)
+S512_WIDE_KEYS(u64, uint64_t, TUNE_F64)
+S512_WIDE_KEYS(u32, uint32_t, TUNE_I32)
+#undef S512_WIDE_KEYS
+
 extern "C" s512_status s512kv_sort_i32_dev(int32_t* d_k, int32_t* d_v, size_t n, void* stream) {
     return s512kv_sort_i32_dev_ex(d_k, d_v, n, nullptr, nullptr, stream);
 }

@@ -52,6 +52,15 @@ _SIGS = {
     "s512kv_sort_i32": ([_vp, _vp, _c_size], ctypes.c_int),
     "s512kv_sort_i32_dev": ([_vp, _vp, _c_size, _vp], ctypes.c_int),
     "s512_sort_i32_dev_ex": ([_vp, _c_size, _vp, _vp, _vp], ctypes.c_int),
+    "s512_sort_i64": ([_vp, _c_size], ctypes.c_int),
+    "s512_sort_u64": ([_vp, _c_size], ctypes.c_int),
+    "s512_sort_u32": ([_vp, _c_size], ctypes.c_int),
+    "s512_sort_i64_dev": ([_vp, _c_size, _vp], ctypes.c_int),
+    "s512_sort_u64_dev": ([_vp, _c_size, _vp], ctypes.c_int),
+    "s512_sort_u32_dev": ([_vp, _c_size, _vp], ctypes.c_int),
+    "s512_sort_i64_dev_ex": ([_vp, _c_size, _vp, _vp, _vp], ctypes.c_int),
+    "s512_sort_u64_dev_ex": ([_vp, _c_size, _vp, _vp, _vp], ctypes.c_int),
+    "s512_sort_u32_dev_ex": ([_vp, _c_size, _vp, _vp, _vp], ctypes.c_int),
     "s512_sort_f64_dev_ex": ([_vp, _c_size, _vp, _vp, _vp], ctypes.c_int),
     "s512kv_sort_i32_dev_ex": ([_vp, _vp, _c_size, _vp, _vp, _vp], ctypes.c_int),
     "s512_sort_merge_i32": ([_vp, _c_size], ctypes.c_int),
@@ -131,7 +140,7 @@ SYNTH = {"uniform": 0, "normal": 1, "distinct": 2, "sorted": 3, "reversed": 4, "
 
 def fill(t, kind, seed=0, param=0):
     """Device-side synthetic input (include/s512_synth.h) into a torch tensor."""
-    k = _kind(t)
+    k = _ref_kind(t)
     _check(getattr(lib, f"s512x_fill_{k}")(_ptr(t), t.numel(), SYNTH[kind], seed, param, _stream_of(t)))
     return t
 
@@ -202,17 +211,28 @@ def _stream_of(t):
     return ctypes.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
 
 
-_DT = {np.dtype(np.int32): "i32", np.dtype(np.float64): "f64"}
+_DT = {np.dtype(np.int32): "i32", np.dtype(np.float64): "f64", np.dtype(np.int64): "i64",
+       np.dtype(np.uint64): "u64", np.dtype(np.uint32): "u32"}
 
 
 def _kind(x):
     if _is_torch(x):
         import torch
-        k = {torch.int32: "i32", torch.float64: "f64"}.get(x.dtype)
+        k = {torch.int32: "i32", torch.float64: "f64", torch.int64: "i64", torch.uint64: "u64",
+             torch.uint32: "u32"}.get(x.dtype)
     else:
         k = _DT.get(np.dtype(x.dtype))
     if k is None:
-        raise TypeError(f"unsupported dtype {x.dtype}: the reference API covers int32 and float64")
+        raise TypeError(f"unsupported dtype {x.dtype}: sorts cover int32 and float64 (the reference API) and "
+                        f"int64, uint64, uint32 keys")
+    return k
+
+
+def _ref_kind(x):
+    """Kind of an array for the entry points that cover the reference's key types only."""
+    k = _kind(x)
+    if k not in ("i32", "f64"):
+        raise TypeError(f"unsupported dtype {x.dtype}: this entry point covers int32 and float64 (the reference API)")
     return k
 
 
@@ -262,7 +282,7 @@ def _dev(x):
 
 def sort_merge(a):
     """Sort512::SortOmpMerge / SortOmpMergeDeps / SortOmpParMerge: GPU merge sort."""
-    k = _kind(a)
+    k = _ref_kind(a)
     if _dev(a):
         _check(getattr(lib, f"s512_sort_merge_{k}_dev")(_ptr(a), _n(a), _stream_of(a)))
     else:
@@ -272,7 +292,7 @@ def sort_merge(a):
 
 def partition(a, left, right, pivot):
     """Sort512::Partition512: returns left + #{a[left..right] <= pivot}."""
-    k = _kind(a)
+    k = _ref_kind(a)
     out = _c_size()
     cast = int if k == "i32" else float
     if _dev(a):
@@ -296,7 +316,7 @@ def kv_partition(keys, values, left, right, pivot):
 
 def smallsort16v(a):
     """Sort512::SmallSort16V (<= 256 int32 / <= 128 float64 keys)."""
-    _check(getattr(lib, f"s512_smallsort16v_{_kind(a)}")(_ptr(a), _n(a)))
+    _check(getattr(lib, f"s512_smallsort16v_{_ref_kind(a)}")(_ptr(a), _n(a)))
     return a
 
 
@@ -308,7 +328,7 @@ def kv_smallsort16v(keys, values):
 
 def merge_inplace(a, center):
     """ParallelInplace::parallelMergeInPlace: merge sorted a[:center] and a[center:]."""
-    k = _kind(a)
+    k = _ref_kind(a)
     if _dev(a):
         _check(getattr(lib, f"s512_merge_inplace_{k}_dev")(_ptr(a), _n(a), center, _stream_of(a)))
     else:
@@ -332,7 +352,7 @@ def _offsets_ok(keys, offsets):
 
 def segsort(keys, offsets, stats=False, options=None):
     """Sort every segment keys[offsets[i]:offsets[i+1]] independently, in place."""
-    k = _kind(keys)
+    k = _ref_kind(keys)
     _offsets_ok(keys, offsets)
     nseg = _n(offsets) - 1
     if _dev(keys):
@@ -351,7 +371,7 @@ class SegsortPlan:
     """A captured segmented sort for fixed device tensors (keys, offsets)."""
 
     def __init__(self, keys, offsets):
-        k = _kind(keys)
+        k = _ref_kind(keys)
         _offsets_ok(keys, offsets)
         if not _dev(keys):
             raise ValueError("a plan needs device tensors")

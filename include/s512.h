@@ -72,6 +72,19 @@ s512_status s512_sort_merge_f64(double* a, size_t n);
 s512_status s512_sort_merge_i32_dev(int32_t* d_a, size_t n, void* stream);
 s512_status s512_sort_merge_f64_dev(double* d_a, size_t n, void* stream);
 
+/* ------------------------------------------------ 64-bit and unsigned keys */
+/* No reference counterpart (the reference sorts int and double keys): keys-only
+ * ascending sorts of int64 (signed order), uint64 and uint32 by the same
+ * engine, for callers whose keys are wider or unsigned.  The _ex variants take
+ * the options and statistics of s512_sort_*_dev_ex below (the learned
+ * classifier is not offered for these types).                              */
+s512_status s512_sort_i64(int64_t* a, size_t n);
+s512_status s512_sort_u64(uint64_t* a, size_t n);
+s512_status s512_sort_u32(uint32_t* a, size_t n);
+s512_status s512_sort_i64_dev(int64_t* d_a, size_t n, void* stream);
+s512_status s512_sort_u64_dev(uint64_t* d_a, size_t n, void* stream);
+s512_status s512_sort_u32_dev(uint32_t* d_a, size_t n, void* stream);
+
 /* ------------------------------------------------------------- partition */
 /* Sort512::Partition512<size_t>(int[], left, right, pivot)    sort512.hpp:5729
  * Sort512::Partition512<size_t>(double[], left, right, pivot) sort512.hpp:5821
@@ -169,6 +182,10 @@ typedef struct s512_stats {
 /* Sort512::Sort with options and statistics (device pointers). */
 s512_status s512_sort_i32_dev_ex(int32_t* d_a, size_t n, const s512_options* opt, s512_stats* stats, void* stream);
 s512_status s512_sort_f64_dev_ex(double* d_a, size_t n, const s512_options* opt, s512_stats* stats, void* stream);
+/* The same for the 64-bit and unsigned keys above (no reference counterpart). */
+s512_status s512_sort_i64_dev_ex(int64_t* d_a, size_t n, const s512_options* opt, s512_stats* stats, void* stream);
+s512_status s512_sort_u64_dev_ex(uint64_t* d_a, size_t n, const s512_options* opt, s512_stats* stats, void* stream);
+s512_status s512_sort_u32_dev_ex(uint32_t* d_a, size_t n, const s512_options* opt, s512_stats* stats, void* stream);
 
 /* Learned partitioner model check: full-batch loss and gradient (fp32) of the
  * monotone 1-32-32-1 CDF model at parameters d_params (s512_mlp_nparams()
