@@ -53,10 +53,10 @@ s512_status with_staged_array(T* a, size_t n, Fn fn) {
     return hs.out(0, a, n * sizeof(T));
 }
 
-// The same for a key array and a value array of n int32 each (slots 0 and
-// 1, under one guard: a guard must never be nested on one device).
-template <class Fn>
-s512_status with_staged_pair(int32_t* k, int32_t* v, size_t n, Fn fn) {
+// The same for a key array and an int32 value array of n elements each
+// (slots 0 and 1, under one guard: a guard must never be nested on one device).
+template <class K, class Fn>
+s512_status with_staged_pair(K* k, int32_t* v, size_t n, Fn fn) {
     if (!k || !v) {
         set_error("null array pointer");
         return S512_ERR_INVALID;
@@ -66,16 +66,16 @@ s512_status with_staged_pair(int32_t* k, int32_t* v, size_t n, Fn fn) {
     HostStage hs;
     void* dk = k;
     void* dv = v;
-    const size_t bytes = n * sizeof(int32_t);
+    const size_t kbytes = n * sizeof(K), vbytes = n * sizeof(int32_t);
     if (!kdev)
-        if (s512_status rc = hs.in(0, k, bytes, &dk)) return rc;
+        if (s512_status rc = hs.in(0, k, kbytes, &dk)) return rc;
     if (!vdev)
-        if (s512_status rc = hs.in(1, v, bytes, &dv)) return rc;
-    if (s512_status rc = fn((int32_t*)dk, (int32_t*)dv, (hipStream_t)0)) return rc;
+        if (s512_status rc = hs.in(1, v, vbytes, &dv)) return rc;
+    if (s512_status rc = fn((K*)dk, (int32_t*)dv, (hipStream_t)0)) return rc;
     if (!kdev)
-        if (s512_status rc = hs.out(0, k, bytes)) return rc;
+        if (s512_status rc = hs.out(0, k, kbytes)) return rc;
     if (!vdev)
-        if (s512_status rc = hs.out(1, v, bytes)) return rc;
+        if (s512_status rc = hs.out(1, v, vbytes)) return rc;
     return S512_OK;
 }
 

@@ -372,6 +372,66 @@ __global__ __launch_bounds__(P) void k_tile_sort_kv(const TileSeg* segs, const u
     }
 }
 
+// Key/value tiles with 64-bit keys: (key, position) does not fit one 64-bit
+// network key, so the keys are sorted alone and each value is then placed at
+// its key's rank -- the first sorted slot holding an equal key (binary search
+// in LDS) plus a per-slot counter for duplicates.  Equal keys keep their
+// values in no particular order, as in the reference's key/value sort.
+template <class T, int P, int E>
+__global__ __launch_bounds__(P) void k_tile_sort_kv64(const TileSeg* segs, const uint32_t* count, T* ka, int32_t* va,
+                                                      const T* kb, const int32_t* vb) {
+    using Tr = KeyTraits<T>;
+    using BB = BlockBitonic<UintOrder<uint64_t>, P, E>;
+    constexpr uint32_t N = BB::N;
+    __shared__ uint64_t lds[N];
+    __shared__ uint32_t dup[N];
+    __shared__ int32_t lval[N];
+    const uint32_t cnt = *count;
+    const uint32_t t = threadIdx.x;
+    for (uint32_t s = blockIdx.x; s < cnt; s += gridDim.x) {
+        const TileSeg sg = segs[s];
+        const T* sk = (sg.buf ? kb : ka) + sg.begin;
+        const int32_t* sv = (sg.buf ? vb : va) + sg.begin;
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const uint32_t i = m * P + t;
+            const uint64_t x = Tr::enc(sk[i < sg.size ? i : sg.size - 1]);
+            lds[BB::phys(i)] = i < sg.size ? x : ~0ull;
+            dup[i] = 0;
+        }
+        uint64_t v[E];
+        BB::from_striped(v, lds, t);
+        BB::sort(v, lds, t);
+        BB::to_striped(v, lds, t);
+        // pads (all ones) sort after every key below all ones, so a key's
+        // lower bound plus its duplicate count stays below sg.size
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const uint32_t i = m * P + t;
+            if (i < sg.size) {
+                const uint64_t x = Tr::enc(sk[i]);
+                uint32_t lo = 0, hi = N;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (lds[BB::phys(mid)] < x) lo = mid + 1;
+                    else hi = mid;
+                }
+                lval[lo + atomicAdd(&dup[lo], 1u)] = sv[i];
+            }
+        }
+        __syncthreads();  // every source key and value read before any is overwritten
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const uint32_t i = m * P + t;
+            if (i < sg.size) {
+                ka[sg.begin + i] = Tr::dec(lds[BB::phys(i)]);
+                va[sg.begin + i] = lval[i];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 template <int E>
 __global__ __launch_bounds__(256) void k_wave_sort_kv(const TileSeg* segs, const uint32_t* count, int32_t* ka,
                                                       int32_t* va, const int32_t* kb, const int32_t* vb) {
@@ -1077,7 +1137,11 @@ inline void launch_scatter(int shape, bool use_ids, uint32_t grid, hipStream_t s
         switch (shape) {
             case 1: S512_SCATTER(true, 256, 16, false); break;
             case 2: S512_SCATTER(true, 1024, 8, false); break;
-            case 3: S512_SCATTER(true, 512, 16, true); break;
+            case 3:
+                // the deferred-line buffers do not fit beside 8-byte keys with values: shape 0
+                if constexpr (KV && sizeof(T) == 8) S512_SCATTER(true, 512, 16, false);
+                else S512_SCATTER(true, 512, 16, true);
+                break;
             default: S512_SCATTER(true, 512, 16, false); break;
         }
     } else {

@@ -113,6 +113,23 @@ static void launch_tile_kv(int cls, int plan, const TileSeg* segs, const uint32_
 #undef S512_TKV
 }
 
+// Key/value tiles with 64-bit keys (one shape per class).
+template <class T>
+static void launch_tile_kv64(int cls, const TileSeg* segs, const uint32_t* dcount, uint32_t count, T* ka,
+                             int32_t* va, const T* kb, const int32_t* vb, hipStream_t st) {
+    if (count == 0) return;
+#define S512_TKV64(PP, EE) k_tile_sort_kv64<T, PP, EE><<<count, PP, 0, st>>>(segs, dcount, ka, va, kb, vb)
+    switch (cls) {
+        case 0:
+        case 1: S512_TKV64(64, 8); break;
+        case 2: S512_TKV64(64, 16); break;
+        case 3: S512_TKV64(128, 16); break;
+        case 4: S512_TKV64(256, 16); break;
+        default: S512_TKV64(512, 16); break;
+    }
+#undef S512_TKV64
+}
+
 template <class T, bool KV>
 struct TileCfg {
     static constexpr bool WIDE = KV || sizeof(T) == 8;
@@ -129,7 +146,8 @@ struct TileCfg {
 template <class T, bool KV>
 static void launch_tile(int cls, int plan, const TileSeg* segs, const uint32_t* dcount, uint32_t count, T* a,
                         int32_t* va, const T* b, const int32_t* vb, hipStream_t st, bool f64_fast = false) {
-    if constexpr (KV) launch_tile_kv(cls, plan, segs, dcount, count, a, va, b, vb, st);
+    if constexpr (KV && sizeof(T) == 8) launch_tile_kv64<T>(cls, segs, dcount, count, a, va, b, vb, st);
+    else if constexpr (KV) launch_tile_kv(cls, plan, segs, dcount, count, a, va, b, vb, st);
     else if constexpr (std::is_same<T, double>::value) {
         if (f64_fast) launch_tile_f64_fast(cls, plan, segs, dcount, count, a, b, st);
         else launch_tile_keys<T>(cls, plan, segs, dcount, count, a, b, st);
@@ -532,6 +550,9 @@ template s512_status sort_engine_locked<uint64_t, false>(uint64_t*, int32_t*, si
 template s512_status sort_engine_locked<uint32_t, false>(uint32_t*, int32_t*, size_t, hipStream_t, const EngineParams&, s512_stats*, bool, DeviceState&);
 template s512_status sort_engine<double, false>(double*, int32_t*, size_t, hipStream_t, const EngineParams&, s512_stats*, bool);
 template s512_status sort_engine<int32_t, true>(int32_t*, int32_t*, size_t, hipStream_t, const EngineParams&, s512_stats*, bool);
+template s512_status sort_engine<double, true>(double*, int32_t*, size_t, hipStream_t, const EngineParams&, s512_stats*, bool);
+template s512_status sort_engine<int64_t, true>(int64_t*, int32_t*, size_t, hipStream_t, const EngineParams&, s512_stats*, bool);
+template s512_status sort_engine<uint64_t, true>(uint64_t*, int32_t*, size_t, hipStream_t, const EngineParams&, s512_stats*, bool);
 template s512_status sort_engine<int64_t, false>(int64_t*, int32_t*, size_t, hipStream_t, const EngineParams&, s512_stats*, bool);
 template s512_status sort_engine<uint64_t, false>(uint64_t*, int32_t*, size_t, hipStream_t, const EngineParams&, s512_stats*, bool);
 template s512_status sort_engine<uint32_t, false>(uint32_t*, int32_t*, size_t, hipStream_t, const EngineParams&, s512_stats*, bool);
@@ -720,6 +741,74 @@ S512_WIDE_KEYS(i64, int64_t, TUNE_F64)
 S512_WIDE_KEYS(u64, uint64_t, TUNE_F64)
 S512_WIDE_KEYS(u32, uint32_t, TUNE_I32)
 #undef S512_WIDE_KEYS
+
+// Key/value sorts with 64-bit keys and argsort (no reference counterpart;
+// Sort512kv covers int keys).  64-bit keys take the fp64 tuned table with the
+// scatter shapes whose LDS holds 8-byte keys with their values.
+static EngineParams kv64_params(size_t n) {
+    EngineParams p = params_for(nullptr, TUNE_F64, n);
+    if (p.scatter_shape == 3) p.scatter_shape = 0;  // no deferred-line shape for wide pairs
+    return p;
+}
+#define S512_KV64(NAME, T)                                                                                  \
+    extern "C" s512_status s512kv_sort_##NAME##_dev(T* d_k, int32_t* d_v, size_t n, void* stream) {        \
+        if (n > 1 && (!d_k || !d_v)) { set_error("null array"); return S512_ERR_INVALID; }                 \
+        return sort_engine<T, true>(d_k, d_v, n, (hipStream_t)stream, kv64_params(n), nullptr, false);      \
+    }                                                                                                       \
+    extern "C" s512_status s512kv_sort_##NAME(T* k, int32_t* v, size_t n) {                                 \
+        if (n <= 1) return S512_OK;                                                                         \
+        return with_staged_pair(k, v, n, [n](T* dk, int32_t* dv, hipStream_t st) {                         \
+            return s512kv_sort_##NAME##_dev(dk, dv, n, st);                                                 \
+        });                                                                                                 \
+    }
+S512_KV64(f64, double)
+S512_KV64(i64, int64_t)
+S512_KV64(u64, uint64_t)
+#undef S512_KV64
+
+__global__ void k_iota(int32_t* idx, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        idx[i] = (int32_t)i;
+}
+
+// idx[i] = position in keys of the i-th smallest key.  The keys are copied
+// to the per-device staging buffer (slot 0) and sorted there with the
+// positions as values, so the caller's keys are left unchanged.
+template <class T, bool KV_WIDE>
+static s512_status argsort_any(const T* keys, int32_t* idx, size_t n, hipStream_t st) {
+    if (n == 0) return S512_OK;
+    if (!keys || !idx) { set_error("null array"); return S512_ERR_INVALID; }
+    if (n > 0x7FFFFFFFull) { set_error("argsort of %zu keys: positions are int32", n); return S512_ERR_TOO_LARGE; }
+    const bool kdev = is_device_pointer(keys), idev = is_device_pointer(idx);
+    HostStage hs;
+    void* kc = nullptr;
+    if (s512_status rc = hs.in(0, keys, n * sizeof(T), &kc, !kdev)) return rc;
+    if (kdev) S512_HIP(hipMemcpyAsync(kc, keys, n * sizeof(T), hipMemcpyDeviceToDevice, st));
+    void* di = idx;
+    if (!idev)
+        if (s512_status rc = hs.in(1, nullptr, n * sizeof(int32_t), &di, false)) return rc;
+    k_iota<<<(unsigned)std::min<size_t>((n + 255) / 256, 65536), 256, 0, st>>>((int32_t*)di, n);
+    S512_CHECK_LAUNCH();
+    s512_status rc;
+    if constexpr (KV_WIDE) rc = sort_engine<T, true>((T*)kc, (int32_t*)di, n, st, kv64_params(n), nullptr, false);
+    else rc = sort_engine<T, true>((T*)kc, (int32_t*)di, n, st, params_for(nullptr, TUNE_KV, n), nullptr, false);
+    if (rc) return rc;
+    S512_HIP(hipStreamSynchronize(st));  // the staging buffer is reused by the next call
+    if (!idev) return hs.out(1, idx, n * sizeof(int32_t));
+    return S512_OK;
+}
+extern "C" s512_status s512_argsort_i32(const int32_t* keys, int32_t* idx, size_t n, void* stream) {
+    return argsort_any<int32_t, false>(keys, idx, n, (hipStream_t)stream);
+}
+extern "C" s512_status s512_argsort_f64(const double* keys, int32_t* idx, size_t n, void* stream) {
+    return argsort_any<double, true>(keys, idx, n, (hipStream_t)stream);
+}
+extern "C" s512_status s512_argsort_i64(const int64_t* keys, int32_t* idx, size_t n, void* stream) {
+    return argsort_any<int64_t, true>(keys, idx, n, (hipStream_t)stream);
+}
+extern "C" s512_status s512_argsort_u64(const uint64_t* keys, int32_t* idx, size_t n, void* stream) {
+    return argsort_any<uint64_t, true>(keys, idx, n, (hipStream_t)stream);
+}
 
 extern "C" s512_status s512kv_sort_i32_dev(int32_t* d_k, int32_t* d_v, size_t n, void* stream) {
     return s512kv_sort_i32_dev_ex(d_k, d_v, n, nullptr, nullptr, stream);

@@ -52,6 +52,16 @@ _SIGS = {
     "s512kv_sort_i32": ([_vp, _vp, _c_size], ctypes.c_int),
     "s512kv_sort_i32_dev": ([_vp, _vp, _c_size, _vp], ctypes.c_int),
     "s512_sort_i32_dev_ex": ([_vp, _c_size, _vp, _vp, _vp], ctypes.c_int),
+    "s512kv_sort_f64": ([_vp, _vp, _c_size], ctypes.c_int),
+    "s512kv_sort_i64": ([_vp, _vp, _c_size], ctypes.c_int),
+    "s512kv_sort_u64": ([_vp, _vp, _c_size], ctypes.c_int),
+    "s512kv_sort_f64_dev": ([_vp, _vp, _c_size, _vp], ctypes.c_int),
+    "s512kv_sort_i64_dev": ([_vp, _vp, _c_size, _vp], ctypes.c_int),
+    "s512kv_sort_u64_dev": ([_vp, _vp, _c_size, _vp], ctypes.c_int),
+    "s512_argsort_i32": ([_vp, _vp, _c_size, _vp], ctypes.c_int),
+    "s512_argsort_f64": ([_vp, _vp, _c_size, _vp], ctypes.c_int),
+    "s512_argsort_i64": ([_vp, _vp, _c_size, _vp], ctypes.c_int),
+    "s512_argsort_u64": ([_vp, _vp, _c_size, _vp], ctypes.c_int),
     "s512_sort_i64": ([_vp, _c_size], ctypes.c_int),
     "s512_sort_u64": ([_vp, _c_size], ctypes.c_int),
     "s512_sort_u32": ([_vp, _c_size], ctypes.c_int),
@@ -258,18 +268,42 @@ def sort(a):
 
 
 def kv_sort(keys, values):
-    """Sort512kv::Sort: sort int32 keys ascending, values follow their keys."""
-    if _kind(keys) != "i32" or _kind(values) != "i32":
-        raise TypeError("Sort512kv covers int32 keys and int32 values")
+    """Sort512kv::Sort: sort keys ascending, int32 values follow their keys.
+    int32 keys are the reference's layout; float64, int64 and uint64 keys are
+    an extension."""
+    k = _kind(keys)
+    if _kind(values) != "i32" or k == "u32":
+        raise TypeError("key/value sorts take int32 values and int32, float64, int64 or uint64 keys")
     n = keys.numel() if _is_torch(keys) else keys.size
     m = values.numel() if _is_torch(values) else values.size
     if n != m:
         raise ValueError("keys and values differ in length")
     if _is_torch(keys) and keys.is_cuda:
-        _check(lib.s512kv_sort_i32_dev(_ptr(keys), _ptr(values), n, _stream_of(keys)))
+        _check(getattr(lib, f"s512kv_sort_{k}_dev")(_ptr(keys), _ptr(values), n, _stream_of(keys)))
     else:
-        _check(lib.s512kv_sort_i32(_ptr(keys), _ptr(values), n))
+        _check(getattr(lib, f"s512kv_sort_{k}")(_ptr(keys), _ptr(values), n))
     return keys, values
+
+
+def argsort(keys, out=None):
+    """Positions that sort `keys` ascending (int32; keys unchanged; equal keys
+    in no particular order).  `out` (int32, same length) may be given; by
+    default it is allocated beside the keys (device tensor or numpy array)."""
+    k = _kind(keys)
+    if k == "u32":
+        raise TypeError("argsort covers int32, float64, int64 and uint64 keys")
+    n = _n(keys)
+    if out is None:
+        if _is_torch(keys):
+            import torch
+            out = torch.empty(n, dtype=torch.int32, device=keys.device)
+        else:
+            out = np.empty(n, dtype=np.int32)
+    if _kind(out) != "i32" or _n(out) != n:
+        raise ValueError("out must be int32 of the keys' length")
+    stream = _stream_of(keys) if _dev(keys) else None
+    _check(getattr(lib, f"s512_argsort_{k}")(_ptr(keys), _ptr(out), n, stream))
+    return out
 
 
 def _n(x):

@@ -85,6 +85,25 @@ s512_status s512_sort_i64_dev(int64_t* d_a, size_t n, void* stream);
 s512_status s512_sort_u64_dev(uint64_t* d_a, size_t n, void* stream);
 s512_status s512_sort_u32_dev(uint32_t* d_a, size_t n, void* stream);
 
+/* ---------------------------------- key/value with 64-bit keys; argsort */
+/* No reference counterpart (Sort512kv covers int keys, sort512kv.hpp:6132):
+ * keys ascending, each int32 value moves with its key; equal keys keep their
+ * values in no particular order, as in Sort512kv.  Host or device pointers. */
+s512_status s512kv_sort_f64(double* k, int32_t* v, size_t n);
+s512_status s512kv_sort_i64(int64_t* k, int32_t* v, size_t n);
+s512_status s512kv_sort_u64(uint64_t* k, int32_t* v, size_t n);
+s512_status s512kv_sort_f64_dev(double* d_k, int32_t* d_v, size_t n, void* stream);
+s512_status s512kv_sort_i64_dev(int64_t* d_k, int32_t* d_v, size_t n, void* stream);
+s512_status s512kv_sort_u64_dev(uint64_t* d_k, int32_t* d_v, size_t n, void* stream);
+
+/* argsort: idx[i] = position in keys of the i-th smallest key (equal keys in
+ * no particular order).  keys are not modified; n < 2^31.  keys and idx may
+ * each be host or device memory; device work runs on `stream`.             */
+s512_status s512_argsort_i32(const int32_t* keys, int32_t* idx, size_t n, void* stream);
+s512_status s512_argsort_f64(const double* keys, int32_t* idx, size_t n, void* stream);
+s512_status s512_argsort_i64(const int64_t* keys, int32_t* idx, size_t n, void* stream);
+s512_status s512_argsort_u64(const uint64_t* keys, int32_t* idx, size_t n, void* stream);
+
 /* ------------------------------------------------------------- partition */
 /* Sort512::Partition512<size_t>(int[], left, right, pivot)    sort512.hpp:5729
  * Sort512::Partition512<size_t>(double[], left, right, pivot) sort512.hpp:5821

@@ -99,3 +99,12 @@ def test_header_rejects_other_types(tmp_path):
     r = subprocess.run(["g++", "-std=c++11", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), str(src)],
                        capture_output=True, text=True)
     assert r.returncode != 0 and "sorts int and double" in r.stderr
+
+
+def test_argsort_rejects_more_than_int32_positions(s512):
+    """Positions are int32: n >= 2^31 is refused before any memory is touched."""
+    import ctypes
+    a = (ctypes.c_double * 1)()
+    i = (ctypes.c_int32 * 1)()
+    rc = s512.lib.s512_argsort_f64(ctypes.cast(a, ctypes.c_void_p), ctypes.cast(i, ctypes.c_void_p), 1 << 31, None)
+    assert rc == 5  # S512_ERR_TOO_LARGE
