@@ -65,7 +65,8 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=1 << 25)
     ap.add_argument("--workload", default="plain", choices=list(WORKLOADS),
                     help="plain: the 256M fp64 headline (default); the others time the secondary workloads "
-                         "(256M int32, host-pointer end to end, key/value Zipf, segmented batches, learned 1G) "
+                         "(256M int32, host-pointer end to end, int64 keys and fp64 argsort, key/value Zipf, "
+                         "segmented batches, learned 1G) "
                          "and print their own JSON line")
     return ap.parse_args()
 
@@ -455,7 +456,35 @@ def bench_host(args):
                  {"verified": ok, "pcie_bound_ms": pcie_ms, "ratio_to_pcie_bound": t / pcie_ms})
 
 
-WORKLOADS = {"plain": None, "plain-i32": bench_plain_i32, "host": bench_host, "kv-zipf": bench_kv_zipf, "segsort-i32": lambda a: bench_segsort(a, "int32"),
+def bench_wide(args):
+    """268,435,456 int64 keys (full range) sorted, and argsort of 268,435,456
+    fp64 keys (standard normal): int32 positions of the ascending order."""
+    import torch
+    import s512
+    n = args.n
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(5)
+    k64 = torch.randint(-(1 << 63), (1 << 63) - 1, (n,), dtype=torch.int64, device=dev, generator=g)
+    w = torch.empty_like(k64)
+    h0 = multiset_hash(k64)
+    ms_i64 = _timed(lambda: s512.sort(w), lambda: w.copy_(k64), args.steps, args.warmup)
+    ok = is_sorted(w) and multiset_hash(w) == h0
+    del k64, w
+    f = torch.empty(n, dtype=torch.float64, device=dev)
+    s512.fill(f, "normal", seed=1)
+    idx = torch.empty(n, dtype=torch.int32, device=dev)
+    ms_arg = _timed(lambda: s512.argsort(f, out=idx), lambda: None, args.steps, args.warmup)
+    ok = ok and is_sorted(f[idx.long()])
+    t_i64, t_arg = sum(ms_i64) / len(ms_i64), sum(ms_arg) / len(ms_arg)
+    return _line("sorted keys/s, 256M int64 keys", "keys/s", n, ms_i64, args.steps, args.warmup, "int64",
+                 "synthetic (torch.randint full int64 range; fp64 normal for argsort)",
+                 {"workload": f"int64 key sort and fp64 argsort, {n} keys each"},
+                 {"verified": ok, "int64_sort_ms": t_i64, "argsort_f64_ms": t_arg,
+                  "argsort_f64_keys_per_s": n / (t_arg / 1e3)})
+
+
+WORKLOADS = {"plain": None, "plain-i32": bench_plain_i32, "host": bench_host, "wide": bench_wide, "kv-zipf": bench_kv_zipf, "segsort-i32": lambda a: bench_segsort(a, "int32"),
              "segsort-f64": lambda a: bench_segsort(a, "float64"), "learned-1g": bench_learned}
 
 
