@@ -62,6 +62,14 @@ _SIGS = {
     "s512_argsort_f64": ([_vp, _vp, _c_size, _vp], ctypes.c_int),
     "s512_argsort_i64": ([_vp, _vp, _c_size, _vp], ctypes.c_int),
     "s512_argsort_u64": ([_vp, _vp, _c_size, _vp], ctypes.c_int),
+    "s512_topk_i32": ([_vp, _c_size, _c_size, _vp, _vp], ctypes.c_int),
+    "s512_topk_f64": ([_vp, _c_size, _c_size, _vp, _vp], ctypes.c_int),
+    "s512_topk_i64": ([_vp, _c_size, _c_size, _vp, _vp], ctypes.c_int),
+    "s512_topk_u64": ([_vp, _c_size, _c_size, _vp, _vp], ctypes.c_int),
+    "s512_select_i32": ([_vp, _c_size, _c_size, _vp, _vp], ctypes.c_int),
+    "s512_select_f64": ([_vp, _c_size, _c_size, _vp, _vp], ctypes.c_int),
+    "s512_select_i64": ([_vp, _c_size, _c_size, _vp, _vp], ctypes.c_int),
+    "s512_select_u64": ([_vp, _c_size, _c_size, _vp, _vp], ctypes.c_int),
     "s512_sort_i64": ([_vp, _c_size], ctypes.c_int),
     "s512_sort_u64": ([_vp, _c_size], ctypes.c_int),
     "s512_sort_u32": ([_vp, _c_size], ctypes.c_int),
@@ -283,6 +291,38 @@ def kv_sort(keys, values):
     else:
         _check(getattr(lib, f"s512kv_sort_{k}")(_ptr(keys), _ptr(values), n))
     return keys, values
+
+
+def topk(keys, k, out=None):
+    """The k smallest keys in ascending order (keys unchanged).  `out` (same
+    dtype, length k) may be given; by default it is allocated beside the keys."""
+    kd = _kind(keys)
+    if kd == "u32":
+        raise TypeError("topk covers int32, float64, int64 and uint64 keys")
+    n = _n(keys)
+    if out is None:
+        if _is_torch(keys):
+            import torch
+            out = torch.empty(k, dtype=keys.dtype, device=keys.device)
+        else:
+            out = np.empty(k, dtype=keys.dtype)
+    if _kind(out) != kd or _n(out) != k:
+        raise ValueError("out must have the keys' dtype and length k")
+    stream = _stream_of(keys) if _dev(keys) else None
+    _check(getattr(lib, f"s512_topk_{kd}")(_ptr(keys), n, k, _ptr(out), stream))
+    return out
+
+
+def select(keys, k):
+    """The k-th smallest key (0-based), as a Python number."""
+    kd = _kind(keys)
+    if kd == "u32":
+        raise TypeError("select covers int32, float64, int64 and uint64 keys")
+    ct = {"i32": ctypes.c_int32, "f64": ctypes.c_double, "i64": ctypes.c_int64, "u64": ctypes.c_uint64}[kd]
+    v = ct()
+    stream = _stream_of(keys) if _dev(keys) else None
+    _check(getattr(lib, f"s512_select_{kd}")(_ptr(keys), _n(keys), k, ctypes.cast(ctypes.byref(v), _vp), stream))
+    return v.value
 
 
 def argsort(keys, out=None):

@@ -65,7 +65,7 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=1 << 25)
     ap.add_argument("--workload", default="plain", choices=list(WORKLOADS),
                     help="plain: the 256M fp64 headline (default); the others time the secondary workloads "
-                         "(256M int32, host-pointer end to end, int64 keys and fp64 argsort, key/value Zipf, "
+                         "(256M int32, host-pointer end to end, int64 keys and fp64 argsort, top-k, key/value Zipf, "
                          "segmented batches, learned 1G) "
                          "and print their own JSON line")
     return ap.parse_args()
@@ -484,7 +484,32 @@ def bench_wide(args):
                   "argsort_f64_keys_per_s": n / (t_arg / 1e3)})
 
 
-WORKLOADS = {"plain": None, "plain-i32": bench_plain_i32, "host": bench_host, "wide": bench_wide, "kv-zipf": bench_kv_zipf, "segsort-i32": lambda a: bench_segsort(a, "int32"),
+def bench_topk(args):
+    """Top-k and selection on 268,435,456 fp64 keys (standard normal): the
+    1,048,576 smallest keys in order, and the median; checked against the
+    full sort's prefix."""
+    import torch
+    import s512
+    n, k = args.n, 1 << 20
+    dev = torch.device("cuda:0")
+    a = torch.empty(n, dtype=torch.float64, device=dev)
+    s512.fill(a, "normal", seed=1)
+    out = torch.empty(k, dtype=torch.float64, device=dev)
+    ms_topk = _timed(lambda: s512.topk(a, k, out=out), lambda: None, args.steps, args.warmup)
+    med = {}
+    ms_sel = _timed(lambda: med.update(v=s512.select(a, n // 2)), lambda: None, args.steps, args.warmup)
+    full = a.clone()
+    s512.sort(full)
+    ok = bool(torch.equal(out.view(torch.int64), full[:k].view(torch.int64))) and med["v"] == float(full[n // 2])
+    t = sum(ms_topk) / len(ms_topk)
+    return _line("keys/s scanned, top-1M of 256M fp64", "keys/s", n, ms_topk, args.steps, args.warmup, "f64",
+                 "synthetic (device-generated normal)",
+                 {"workload": f"top-k, {n} fp64 keys, k = {k}; selection of the median"},
+                 {"verified": ok, "topk_ms": t, "select_median_ms": sum(ms_sel) / len(ms_sel)})
+
+
+WORKLOADS = {"plain": None, "plain-i32": bench_plain_i32, "host": bench_host, "wide": bench_wide,
+             "topk": bench_topk, "kv-zipf": bench_kv_zipf, "segsort-i32": lambda a: bench_segsort(a, "int32"),
              "segsort-f64": lambda a: bench_segsort(a, "float64"), "learned-1g": bench_learned}
 
 

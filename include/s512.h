@@ -104,6 +104,20 @@ s512_status s512_argsort_f64(const double* keys, int32_t* idx, size_t n, void* s
 s512_status s512_argsort_i64(const int64_t* keys, int32_t* idx, size_t n, void* stream);
 s512_status s512_argsort_u64(const uint64_t* keys, int32_t* idx, size_t n, void* stream);
 
+/* ------------------------------------------------------ top-k, selection */
+/* No reference counterpart (built from the multiway sort for callers that need
+ * a prefix of the order).  topk: out[0..k) = the k smallest keys, ascending
+ * (0 <= k <= n).  select: *kth = the k-th smallest key, 0-based (k < n).
+ * Keys are not modified; keys and out may each be host or device memory.  */
+s512_status s512_topk_i32(const int32_t* keys, size_t n, size_t k, int32_t* out, void* stream);
+s512_status s512_topk_f64(const double* keys, size_t n, size_t k, double* out, void* stream);
+s512_status s512_topk_i64(const int64_t* keys, size_t n, size_t k, int64_t* out, void* stream);
+s512_status s512_topk_u64(const uint64_t* keys, size_t n, size_t k, uint64_t* out, void* stream);
+s512_status s512_select_i32(const int32_t* keys, size_t n, size_t k, int32_t* kth, void* stream);
+s512_status s512_select_f64(const double* keys, size_t n, size_t k, double* kth, void* stream);
+s512_status s512_select_i64(const int64_t* keys, size_t n, size_t k, int64_t* kth, void* stream);
+s512_status s512_select_u64(const uint64_t* keys, size_t n, size_t k, uint64_t* kth, void* stream);
+
 /* ------------------------------------------------------------- partition */
 /* Sort512::Partition512<size_t>(int[], left, right, pivot)    sort512.hpp:5729
  * Sort512::Partition512<size_t>(double[], left, right, pivot) sort512.hpp:5821

@@ -108,3 +108,14 @@ def test_argsort_rejects_more_than_int32_positions(s512):
     i = (ctypes.c_int32 * 1)()
     rc = s512.lib.s512_argsort_f64(ctypes.cast(a, ctypes.c_void_p), ctypes.cast(i, ctypes.c_void_p), 1 << 31, None)
     assert rc == 5  # S512_ERR_TOO_LARGE
+
+
+def test_topk_select_argument_checks_on_the_host(s512):
+    """k > n (top-k) and k >= n (selection) are refused before any GPU call."""
+    import ctypes
+    a = (ctypes.c_double * 4)()
+    o = (ctypes.c_double * 8)()
+    pa, po = ctypes.cast(a, ctypes.c_void_p), ctypes.cast(o, ctypes.c_void_p)
+    assert s512.lib.s512_topk_f64(pa, 4, 5, po, None) == 1  # S512_ERR_INVALID
+    assert s512.lib.s512_select_f64(pa, 4, 4, po, None) == 1
+    assert s512.lib.s512_topk_f64(pa, 4, 0, po, None) == 0  # k = 0: nothing to do
