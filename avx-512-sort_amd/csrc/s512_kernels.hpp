@@ -737,14 +737,14 @@ __global__ __launch_bounds__(CNT_P) void k_count(const PartSeg* ps, const ChunkR
     bool odd = false;
     // clamped indices keep every load in bounds and unconditional; the buffers
     // hold the raw bits, encoded only where a batch is classified, so that no
-    // instruction right after the prefetch has to wait for it.  Non-temporal:
-    // the keys are read once here (measured 4% faster); the scatter's loads
-    // stay ordinary, where L2 residency measured faster.
+    // instruction right after the prefetch has to wait for it.  (Non-temporal
+    // loads measured 5% faster here but slowed the scatter that follows by
+    // more: ordinary loads.)
     auto load = [&](uint64_t base, K (&d)[CNT_U]) {
 #pragma unroll
         for (int u = 0; u < CNT_U; ++u) {
             const uint64_t i = base + (uint64_t)u * CNT_P + t;
-            d[u] = Tr::bits(__builtin_nontemporal_load(&src[i < hi ? i : hi - 1]));
+            d[u] = Tr::bits(src[i < hi ? i : hi - 1]);
         }
     };
     auto batch = [&](uint64_t base, K (&xr)[CNT_U], K (&xn)[CNT_U]) {
@@ -948,7 +948,15 @@ __global__ __launch_bounds__(P) void k_scatter(const PartSeg* ps, const ChunkRef
     __shared__ L sm;
 
     for (uint32_t wi = blockIdx.x; wi < *nchunk; wi += gridDim.x) {
-    const ChunkRef c = cr[wi];
+    uint32_t wc = wi;
+    if (gridDim.x >= *nchunk) {
+        // workgroups reach the 8 XCDs round-robin: give XCD x a contiguous run of
+        // chunks, so the line two neighbouring chunks share in a bucket is
+        // written through one L2
+        const uint32_t N = *nchunk, q = N / 8, r = N % 8, x = wi % 8;
+        wc = x * q + (x < r ? x : r) + wi / 8;
+    }
+    const ChunkRef c = cr[wc];
     const PartSeg s = ps[c.seg];
     const uint32_t B = 2u << s.logf;
     const uint32_t t = threadIdx.x, lane = t & 63u, wid = t >> 6;
