@@ -153,6 +153,7 @@ s512_status select_band(const T* a, size_t n, size_t r0, size_t r1, hipStream_t 
     using K = typename KeyTraits<T>::K;
     int dev;
     if (s512_status rc = current_device(&dev)) return rc;
+    if (dev < 0 || dev >= 64) { set_error("device %d out of range", dev); return S512_ERR_INVALID; }
     static std::mutex mu[64];
     std::lock_guard<std::mutex> lk(mu[dev]);
     SelScratch<T>& sc = sel_scratch<T>(dev);
