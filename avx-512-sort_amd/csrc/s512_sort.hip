@@ -797,6 +797,63 @@ static s512_status argsort_any(const T* keys, int32_t* idx, size_t n, hipStream_
     if (!idev) return hs.out(1, idx, n * sizeof(int32_t));
     return S512_OK;
 }
+// Key/value sorts with an 8-byte value payload: the keys are sorted with their
+// int32 positions (the kv engine), then the values are gathered through that
+// permutation -- about the HBM traffic of carrying 8-byte values through every
+// partition pass, with the tested 4-byte-value machinery.
+__global__ void k_gather64(int64_t* out, const int64_t* in, const int32_t* idx, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        out[i] = in[idx[i]];
+}
+
+template <class T, bool KV_WIDE>
+static s512_status kv_sort_v64(T* keys, int64_t* vals, size_t n, hipStream_t st) {
+    if (n <= 1) return S512_OK;
+    if (!keys || !vals) { set_error("null array"); return S512_ERR_INVALID; }
+    if (n > 0x7FFFFFFFull) { set_error("%zu pairs: the permutation is int32", n); return S512_ERR_TOO_LARGE; }
+    const bool kdev = is_device_pointer(keys), vdev = is_device_pointer(vals);
+    HostStage hs;
+    void* dk = keys;
+    void* dv = vals;
+    if (!kdev)
+        if (s512_status rc = hs.in(0, keys, n * sizeof(T), &dk)) return rc;
+    if (!vdev)
+        if (s512_status rc = hs.in(1, vals, n * sizeof(int64_t), &dv)) return rc;
+    const size_t ib = (n * sizeof(int32_t) + 255) & ~(size_t)255;
+    void* ws = nullptr;  // positions, then the gathered values
+    if (s512_status rc = hs.in(2, nullptr, ib + n * sizeof(int64_t), &ws, false)) return rc;
+    int32_t* idx = (int32_t*)ws;
+    int64_t* tmp = (int64_t*)((char*)ws + ib);
+    const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+    k_iota<<<grid, 256, 0, st>>>(idx, n);
+    S512_CHECK_LAUNCH();
+    s512_status rc;
+    if constexpr (KV_WIDE) rc = sort_engine<T, true>((T*)dk, idx, n, st, kv64_params(n), nullptr, false);
+    else rc = sort_engine<T, true>((T*)dk, idx, n, st, params_for(nullptr, TUNE_KV, n), nullptr, false);
+    if (rc) return rc;
+    k_gather64<<<grid, 256, 0, st>>>(tmp, (const int64_t*)dv, idx, n);
+    S512_CHECK_LAUNCH();
+    S512_HIP(hipMemcpyAsync(dv, tmp, n * sizeof(int64_t), hipMemcpyDeviceToDevice, st));
+    S512_HIP(hipStreamSynchronize(st));  // the staging buffers are reused by the next call
+    if (!kdev)
+        if (s512_status rc2 = hs.out(0, keys, n * sizeof(T))) return rc2;
+    if (!vdev)
+        if (s512_status rc2 = hs.out(1, vals, n * sizeof(int64_t))) return rc2;
+    return S512_OK;
+}
+extern "C" s512_status s512kv_sort_i32_v64(int32_t* k, int64_t* v, size_t n, void* stream) {
+    return kv_sort_v64<int32_t, false>(k, v, n, (hipStream_t)stream);
+}
+extern "C" s512_status s512kv_sort_f64_v64(double* k, int64_t* v, size_t n, void* stream) {
+    return kv_sort_v64<double, true>(k, v, n, (hipStream_t)stream);
+}
+extern "C" s512_status s512kv_sort_i64_v64(int64_t* k, int64_t* v, size_t n, void* stream) {
+    return kv_sort_v64<int64_t, true>(k, v, n, (hipStream_t)stream);
+}
+extern "C" s512_status s512kv_sort_u64_v64(uint64_t* k, int64_t* v, size_t n, void* stream) {
+    return kv_sort_v64<uint64_t, true>(k, v, n, (hipStream_t)stream);
+}
+
 extern "C" s512_status s512_argsort_i32(const int32_t* keys, int32_t* idx, size_t n, void* stream) {
     return argsort_any<int32_t, false>(keys, idx, n, (hipStream_t)stream);
 }

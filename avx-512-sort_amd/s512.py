@@ -58,6 +58,10 @@ _SIGS = {
     "s512kv_sort_f64_dev": ([_vp, _vp, _c_size, _vp], ctypes.c_int),
     "s512kv_sort_i64_dev": ([_vp, _vp, _c_size, _vp], ctypes.c_int),
     "s512kv_sort_u64_dev": ([_vp, _vp, _c_size, _vp], ctypes.c_int),
+    "s512kv_sort_i32_v64": ([_vp, _vp, _c_size, _vp], ctypes.c_int),
+    "s512kv_sort_f64_v64": ([_vp, _vp, _c_size, _vp], ctypes.c_int),
+    "s512kv_sort_i64_v64": ([_vp, _vp, _c_size, _vp], ctypes.c_int),
+    "s512kv_sort_u64_v64": ([_vp, _vp, _c_size, _vp], ctypes.c_int),
     "s512_argsort_i32": ([_vp, _vp, _c_size, _vp], ctypes.c_int),
     "s512_argsort_f64": ([_vp, _vp, _c_size, _vp], ctypes.c_int),
     "s512_argsort_i64": ([_vp, _vp, _c_size, _vp], ctypes.c_int),
@@ -276,16 +280,24 @@ def sort(a):
 
 
 def kv_sort(keys, values):
-    """Sort512kv::Sort: sort keys ascending, int32 values follow their keys.
-    int32 keys are the reference's layout; float64, int64 and uint64 keys are
-    an extension."""
+    """Sort512kv::Sort: sort keys ascending, values follow their keys.
+    int32 keys with int32 values are the reference's layout; float64, int64
+    and uint64 keys, and 8-byte values (int64 / uint64 / float64), are an
+    extension."""
     k = _kind(keys)
-    if _kind(values) != "i32" or k == "u32":
-        raise TypeError("key/value sorts take int32 values and int32, float64, int64 or uint64 keys")
+    if k == "u32":
+        raise TypeError("key/value sorts take int32, float64, int64 or uint64 keys")
     n = keys.numel() if _is_torch(keys) else keys.size
     m = values.numel() if _is_torch(values) else values.size
     if n != m:
         raise ValueError("keys and values differ in length")
+    vk = _kind(values)
+    if vk in ("i64", "u64", "f64"):
+        stream = _stream_of(keys) if _dev(keys) else None
+        _check(getattr(lib, f"s512kv_sort_{k}_v64")(_ptr(keys), _ptr(values), n, stream))
+        return keys, values
+    if vk != "i32":
+        raise TypeError("values must be int32 or an 8-byte type")
     if _is_torch(keys) and keys.is_cuda:
         _check(getattr(lib, f"s512kv_sort_{k}_dev")(_ptr(keys), _ptr(values), n, _stream_of(keys)))
     else:

@@ -96,6 +96,14 @@ s512_status s512kv_sort_f64_dev(double* d_k, int32_t* d_v, size_t n, void* strea
 s512_status s512kv_sort_i64_dev(int64_t* d_k, int32_t* d_v, size_t n, void* stream);
 s512_status s512kv_sort_u64_dev(uint64_t* d_k, int32_t* d_v, size_t n, void* stream);
 
+/* Key/value sorts with an 8-byte value payload (int64, or any 8-byte value
+ * passed as int64): keys ascending in place, each value follows its key; n <
+ * 2^31.  Host or device pointers; device work runs on `stream`.          */
+s512_status s512kv_sort_i32_v64(int32_t* k, int64_t* v, size_t n, void* stream);
+s512_status s512kv_sort_f64_v64(double* k, int64_t* v, size_t n, void* stream);
+s512_status s512kv_sort_i64_v64(int64_t* k, int64_t* v, size_t n, void* stream);
+s512_status s512kv_sort_u64_v64(uint64_t* k, int64_t* v, size_t n, void* stream);
+
 /* argsort: idx[i] = position in keys of the i-th smallest key (equal keys in
  * no particular order).  keys are not modified; n < 2^31.  keys and idx may
  * each be host or device memory; device work runs on `stream`.             */

@@ -114,3 +114,41 @@ def test_argsort_64m_properties(s512, dev):
     assert bool(torch.all(g[1:] >= g[:-1]).item())
     assert int(idx.to(torch.int64).sum().item()) == n * (n - 1) // 2
     assert bool(torch.equal(torch.sort(idx).values, torch.arange(n, dtype=torch.int32, device=dev)))
+
+
+def _pairs64(k, v):
+    kb = _bits(k)
+    vb = v.view(np.uint64)
+    o = np.lexsort((vb, kb))
+    return kb[o], vb[o]
+
+
+@pytest.mark.parametrize("n", [2, 5000, 1_000_003, 5_000_001])
+@pytest.mark.parametrize("kind", ["i32", "f64", "i64", "u64"])
+def test_kv_8byte_values(s512, orc, dev, kind, n):
+    """8-byte values (arbitrary bit patterns) follow their keys."""
+    rng = np.random.default_rng(12)
+    k = _keys(orc, kind, n, 13)
+    v = rng.integers(-(1 << 63), (1 << 63) - 1, size=n, dtype=np.int64, endpoint=True)
+    tk, tv = to_dev(k, dev), to_dev(v, dev)
+    s512.kv_sort(tk, tv)
+    gk, gv = to_host(tk), to_host(tv)
+    assert np.array_equal(_bits(gk), _bits(_keys_sorted(s512, k)))
+    pk, pv = _pairs64(gk, gv)
+    qk, qv = _pairs64(k, v)
+    assert np.array_equal(pk, qk) and np.array_equal(pv, qv)
+
+
+def test_kv_8byte_values_host_and_mixed(s512, orc, dev):
+    n = 7_000_001
+    k = _keys(orc, "f64", n, 14)
+    v = np.arange(n, dtype=np.float64) * 0.5  # float64 payload passed as 8-byte values
+    k0, v0 = k.copy(), v.copy()
+    s512.kv_sort(k, v)  # both on the host
+    assert np.array_equal(_bits(k), _bits(_keys_sorted(s512, k0)))
+    assert np.array_equal(_bits(k0[(v * 2).astype(np.int64)]), _bits(k))
+    k = k0.copy()
+    tv = to_dev(v0, dev)
+    s512.kv_sort(k, tv)  # keys on the host, values on the device
+    gv = to_host(tv)
+    assert np.array_equal(_bits(k0[(gv * 2).astype(np.int64)]), _bits(k))
