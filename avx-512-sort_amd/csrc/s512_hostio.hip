@@ -210,3 +210,19 @@ s512_status HostStage::out(int slot, void* host, size_t bytes) {
 }
 
 }  // namespace s512
+
+namespace s512 {
+void release_host_stage(int dev) {
+    if (dev < 0 || dev >= MAX_DEV) return;
+    HostStage::State* st;
+    {
+        std::lock_guard<std::mutex> lk(g_states_mu);
+        st = g_states[dev];
+    }
+    if (!st) return;
+    std::lock_guard<std::mutex> lk(st->mu);
+    (void)hipDeviceSynchronize();
+    for (auto& b : st->dev) b.release();
+    // the pinned ring, stream and events stay: they are created once
+}
+}  // namespace s512

@@ -25,6 +25,19 @@ DeviceState& device_state(int dev) {
     return *p;
 }
 
+void release_device_state(int dev) {
+    DeviceState& ds = device_state(dev);
+    std::lock_guard<std::mutex> lk(ds.mu);
+    (void)hipDeviceSynchronize();
+    ds.data.release();
+    ds.arena.release();
+    ds.stage.release();
+    ds.learn.release();
+    ds.ids.release();
+    ds.pinned.release();
+    ds.pinned2.release();
+}
+
 s512_status current_device(int* dev) {
     int n = 0;
     hipError_t e = hipGetDeviceCount(&n);
@@ -50,3 +63,12 @@ bool is_device_pointer(const void* p) {
 
 extern "C" const char* s512_last_error(void) { return s512::last_error(); }
 extern "C" const char* s512_version(void) { return "s512-mi355x 0.1 (gfx950)"; }
+
+extern "C" s512_status s512_release(void) {
+    int dev;
+    if (s512_status rc = s512::current_device(&dev)) return rc;
+    s512::release_host_stage(dev);
+    s512::release_select_scratch(dev);
+    s512::release_device_state(dev);
+    return S512_OK;
+}

@@ -49,6 +49,11 @@ struct DevBuf {
         cap = want;
         return S512_OK;
     }
+    void release() {  // caller has synchronised the device
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
 };
 
 struct HostBuf {
@@ -71,6 +76,11 @@ struct HostBuf {
         }
         cap = want;
         return S512_OK;
+    }
+    void release() {  // caller has synchronised the device
+        if (ptr) (void)hipHostFree(ptr);
+        ptr = nullptr;
+        cap = 0;
     }
 };
 
@@ -128,6 +138,10 @@ struct Profiler {
     }
 };
 DeviceState& device_state(int dev);
+// Frees the cached buffers of the calling device (each module its own).
+void release_device_state(int dev);
+void release_host_stage(int dev);
+void release_select_scratch(int dev);
 s512_status current_device(int* dev);
 
 // Bump allocator over one DevBuf for a level's metadata.

@@ -132,6 +132,7 @@ __global__ __launch_bounds__(256) void k_filter_band(const T* a, uint64_t n, con
 
 template <class T>
 struct SelScratch {
+    std::mutex mu;
     DevBuf smp, bcnt, bstart, keys;
 };
 
@@ -139,6 +140,17 @@ template <class T>
 SelScratch<T>& sel_scratch(int dev) {
     static SelScratch<T> s[64];
     return s[dev];
+}
+
+template <class T>
+void release_sel(int dev) {
+    SelScratch<T>& sc = sel_scratch<T>(dev);
+    std::lock_guard<std::mutex> lk(sc.mu);
+    (void)hipDeviceSynchronize();
+    sc.smp.release();
+    sc.bcnt.release();
+    sc.bstart.release();
+    sc.keys.release();
 }
 
 // Keys of ranks [r0, r1) of a (0 <= r0 < r1 <= n) sorted into scratch: the
@@ -154,9 +166,8 @@ s512_status select_band(const T* a, size_t n, size_t r0, size_t r1, hipStream_t 
     int dev;
     if (s512_status rc = current_device(&dev)) return rc;
     if (dev < 0 || dev >= 64) { set_error("device %d out of range", dev); return S512_ERR_INVALID; }
-    static std::mutex mu[64];
-    std::lock_guard<std::mutex> lk(mu[dev]);
     SelScratch<T>& sc = sel_scratch<T>(dev);
+    std::lock_guard<std::mutex> lk(sc.mu);
     const uint32_t S = (uint32_t)std::min<uint64_t>(n, SEL_SAMPLE);
     const uint64_t nblk = (n + SEL_TILE - 1) / SEL_TILE;
     if (nblk >= (1ull << 31)) { set_error("n = %zu too large", n); return S512_ERR_TOO_LARGE; }
@@ -262,6 +273,15 @@ s512_status select_any(const T* keys, size_t n, size_t k, T* kth, hipStream_t st
 }
 
 }  // namespace
+
+void release_select_scratch(int dev) {
+    if (dev < 0 || dev >= 64) return;
+    release_sel<int32_t>(dev);
+    release_sel<double>(dev);
+    release_sel<int64_t>(dev);
+    release_sel<uint64_t>(dev);
+}
+
 }  // namespace s512
 
 using namespace s512;

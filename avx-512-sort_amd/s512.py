@@ -45,6 +45,7 @@ _vp = ctypes.c_void_p
 _SIGS = {
     "s512_last_error": ([], ctypes.c_char_p),
     "s512_version": ([], ctypes.c_char_p),
+    "s512_release": ([], ctypes.c_int),
     "s512_sort_i32": ([_vp, _c_size], ctypes.c_int),
     "s512_sort_f64": ([_vp, _c_size], ctypes.c_int),
     "s512_sort_i32_dev": ([_vp, _c_size, _vp], ctypes.c_int),
@@ -198,6 +199,11 @@ def declared_symbols():
         for m in re.finditer(r"^\s*(?:s512_status|const char\*|uint32_t)\s+(s512\w+)\s*\(", open(h).read(), re.M):
             names.add(m.group(1))
     return names
+
+
+def release():
+    """Free the current device's cached workspaces (the next call allocates again)."""
+    _check(lib.s512_release())
 
 
 def sort_learned(a, profile=False, **kw):

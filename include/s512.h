@@ -52,6 +52,13 @@ const char* s512_last_error(void);
 /* Library version string. */
 const char* s512_version(void);
 
+/* Frees the calling device's cached workspaces (scratch key arrays, bucket
+ * ids, staging buffers, selection scratch); the next call allocates again.
+ * No reference counterpart (the reference allocates nothing between calls):
+ * for long-running callers that need the memory back.  Other calls on the
+ * device wait for it.                                                      */
+s512_status s512_release(void);
+
 /* ------------------------------------------------------------------ sort */
 /* Sort512::Sort<int,size_t>      sort512.hpp:5960-5963 (CoreSort 5947-5958) */
 s512_status s512_sort_i32(int32_t* a, size_t n);

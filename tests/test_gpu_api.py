@@ -171,3 +171,20 @@ def test_dropin_headers_reference_test_pattern():
     assert os.path.exists(exe), "tests/cpp/shim_test not built (make -C tests/cpp)"
     r = subprocess.run([exe], capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
+
+
+def test_release_and_reuse(s512, orc, dev):
+    """Cached workspaces freed between calls: every kind of call still works after."""
+    import torch
+    a = orc.gen_f64("normal", 3_000_001, 2)
+    want = orc.closed_sort(a.copy())
+    for _ in range(2):
+        t = to_dev(a, dev)
+        s512.sort(t)
+        assert np.array_equal(to_host(t), want)
+        h = a.copy()
+        s512.sort(h)  # host staging
+        assert np.array_equal(h, want)
+        assert np.array_equal(to_host(s512.topk(to_dev(a, dev), 100)), want[:100])
+        s512.release()
+    torch.cuda.synchronize()
