@@ -14,6 +14,12 @@ namespace s512 {
 struct LevelState {
     bool use_learned = false;
     uint32_t lf1 = 0;
+    LearnedModelDev l1{nullptr, nullptr};  // knots placing the first level's splitters
+    LearnedModelDev l2{nullptr, nullptr};  // knots placing the second level's splitters
+    hipEvent_t trained = nullptr;          // l2 is ready once this event completes
+    const float* loss_host = nullptr;      // training loss (pinned), valid after `trained`
+    float max_loss = 0.0f;
+    bool pending = false;                  // the second level still has to wait for `trained`
 };
 
 // The partition level loop over an initial list of segments of the arrays

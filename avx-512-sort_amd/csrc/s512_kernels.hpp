@@ -496,18 +496,33 @@ __global__ __launch_bounds__(256) void k_sample(const PartSeg* ps, const uint32_
 // Written as a sorted pseudo-sample of F keys -- the F - 1 quantiles, then the
 // range's upper key -- which k_build_tree turns into splitters exactly as it
 // does a sample of F keys.
+//
+// `root` is the knot table that placed the root level; `model` places this
+// level.  When they differ (the root from the sample's own CDF, the second
+// level from the trained network over the same knot keys), bucket j's range
+// is mapped into the model's CDF at the root's boundary keys, so the second
+// level subdivides exactly the keys bucket j holds.
 template <class K>
 __global__ __launch_bounds__(256) void k_model_splitters(const PartSeg* ps, const uint32_t* nseg,
-                                                         LearnedModelDev model, K* samples) {
+                                                         LearnedModelDev model, LearnedModelDev root, K* samples) {
     for (uint32_t w = blockIdx.x; w < *nseg; w += gridDim.x) {
         const PartSeg s = ps[w];
         if (!s.mode) continue;
         const uint32_t F = 1u << s.logf;
         const double den = (double)(1ull << s.lshift);
-        const double q0 = (double)s.lbase / den, q1 = (double)(s.lbase + 1) / den;
+        double q0 = (double)s.lbase / den, q1 = (double)(s.lbase + 1) / den;
+        if (s.lshift && root.cdf != model.cdf) {
+            uint32_t lo;
+            double f;
+            model_position(root.cdf, q0, &lo, &f);
+            q0 = s.lbase ? model_cdf_at(model.cdf, lo, f) : 0.0;
+            model_position(root.cdf, q1, &lo, &f);
+            q1 = s.lbase + 1 < (1u << s.lshift) ? model_cdf_at(model.cdf, lo, f) : 1.0;
+        }
         for (uint32_t t = threadIdx.x; t < F; t += blockDim.x) {
             const double q = t + 1 < F ? q0 + (q1 - q0) * (double)(t + 1) / (double)F : q1;
-            samples[s.sample_base + t] = (K)model_quantile(model.key, model.cdf, q);
+            samples[s.sample_base + t] =
+                (K)(model.smp ? model_quantile_rank(model, q) : model_quantile(model.key, model.cdf, q));
         }
     }
 }

@@ -7,7 +7,8 @@
 //                    sample points; parameters initialised so that the net is
 //                    the piecewise-linear interpolation through H+1 sample
 //                    quantiles
-//   k_mlp_train      every Adam step in one workgroup (one point per thread):
+//   k_mlp_train      every Adam step in one workgroup (one point per thread),
+//                    on a stream of its own beside the first partition level:
 //                    forward + backward, the gradient summed in a fixed order
 //                    (deterministic), the update; the final full-batch loss
 //   k_knots          knot keys at equal sample mass, the network's CDF there
@@ -339,13 +340,18 @@ __device__ __forceinline__ float mlp_eval(const float* p, float x) {
 }
 
 // Knot k: the sample key of rank r_k = k (ns - 1) / LEARN_KNOTS and the
-// network's CDF there, kept within half the sample mass of the two adjacent
-// pieces around the empirical value (r_k + 0.5) / ns.
+// network's CDF there, kept within an eighth of the sample mass of the two
+// adjacent pieces (a quarter of a knot's mass) around the empirical value
+// (r_k + 0.5) / ns.  Neighbouring bands cannot overlap, so the knots stay
+// strictly increasing: a flat stretch would place no splitter over a whole
+// knot's keys.  params = nullptr: the empirical value itself.  key = nullptr:
+// keys already written.
 template <class K>
 __global__ __launch_bounds__(256) void k_knots(const K* smp, uint32_t ns, const float* params, const uint64_t* geo,
                                                uint64_t* key, float* cdf_raw) {
     __shared__ float p[MLP_NP];
-    for (int i = threadIdx.x; i < MLP_NP; i += blockDim.x) p[i] = params[i];
+    if (params)
+        for (int i = threadIdx.x; i < MLP_NP; i += blockDim.x) p[i] = params[i];
     __syncthreads();
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k > LEARN_KNOTS) return;
@@ -355,9 +361,9 @@ __global__ __launch_bounds__(256) void k_knots(const K* smp, uint32_t ns, const 
     const uint32_t r = rank(k);
     const uint32_t rl = k > 0 ? rank(k - 1) : 0u, rh = k < LEARN_KNOTS ? rank(k + 1) : ns - 1;
     const float emp = ((float)r + 0.5f) / (float)ns;
-    const float band = 0.5f * (float)(rh - rl) / (float)ns;
-    const float y = mlp_eval(p, learned_feature(smp[r], kmin, inv));
-    key[k] = (uint64_t)smp[r];
+    const float band = 0.125f * (float)(rh - rl) / (float)ns;
+    const float y = params ? mlp_eval(p, learned_feature(smp[r], kmin, inv)) : emp;
+    if (key) key[k] = (uint64_t)smp[r];
     cdf_raw[k] = fminf(fmaxf(fminf(fmaxf(y, emp - band), emp + band), 0.0f), 1.0f);
 }
 
@@ -418,8 +424,7 @@ s512_status build_learned_model(const T* a, size_t n, hipStream_t st, DeviceStat
     sc.take<float>(1);
     sc.take<uint64_t>(2);
     sc.take<uint64_t>(LEARN_KNOTS + 1);
-    sc.take<float>(LEARN_KNOTS + 1);
-    sc.take<float>(LEARN_KNOTS + 1);
+    for (int i = 0; i < 4; ++i) sc.take<float>(LEARN_KNOTS + 1);
     sc.take<uint32_t>(1);
     s512_status rc;
     if ((rc = ds.learn.reserve(sc.off, st))) return rc;
@@ -431,9 +436,18 @@ s512_status build_learned_model(const T* a, size_t n, hipStream_t st, DeviceStat
     float* loss = ar.take<float>(1);
     uint64_t* geo = ar.take<uint64_t>(2);
     uint64_t* key = ar.take<uint64_t>(LEARN_KNOTS + 1);
+    float* ecdf_raw = ar.take<float>(LEARN_KNOTS + 1);
+    float* ecdf = ar.take<float>(LEARN_KNOTS + 1);
     float* cdf_raw = ar.take<float>(LEARN_KNOTS + 1);
     float* cdf = ar.take<float>(LEARN_KNOTS + 1);
     uint32_t* flag = ar.take<uint32_t>(1);
+    if (!ds.lstream) {
+        // default priority; does not synchronise with the null stream
+        S512_HIP(hipStreamCreateWithFlags(&ds.lstream, hipStreamNonBlocking));
+        S512_HIP(hipEventCreateWithFlags(&ds.lready, hipEventDisableTiming));
+        S512_HIP(hipEventCreateWithFlags(&ds.ltrained, hipEventDisableTiming));
+    }
+    if ((rc = ds.lpinned.reserve(256, st))) return rc;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (stats) {
         (void)hipEventCreate(&e0);
@@ -442,49 +456,64 @@ s512_status build_learned_model(const T* a, size_t n, hipStream_t st, DeviceStat
     }
     k_learn_sample<T><<<(ns + 255) / 256, 256, 0, st>>>(a, n, ns, prm.seed ^ 0x1EA2ull, smp);
     S512_CHECK_LAUNCH();
-    // the sample is sorted by this library's own engine (lock already held)
+    // the sample is sorted by this library's own engine (lock already held);
+    // leaves of 8192 keys: one partition level for a sample of a few million
     EngineParams sp = default_params(sizeof(K) == 8);
+    sp.target_leaf = 8192;
     if ((rc = sort_engine_locked<K, false>(smp, nullptr, ns, st, sp, nullptr, false, ds))) return rc;
     k_learn_prep<K><<<1, 256, 0, st>>>(smp, ns, M, u, t, params, geo);
     S512_CHECK_LAUNCH();
     S512_HIP(hipMemsetAsync(flag, 0, sizeof(uint32_t), st));
     k_point_mass<K><<<(ns + 255) / 256, 256, 0, st>>>(smp, ns, run, geo, flag);
     S512_CHECK_LAUNCH();
-    k_mlp_train<<<1, TRAIN_P, 0, st>>>(params, u, t, M, prm.train_steps, prm.learn_rate, loss, nullptr);
+    // the first level's knots: the sample's own CDF
+    k_knots<K><<<(LEARN_KNOTS + 1 + 255) / 256, 256, 0, st>>>(smp, ns, nullptr, geo, key, ecdf_raw);
     S512_CHECK_LAUNCH();
-    k_knots<K><<<(LEARN_KNOTS + 1 + 255) / 256, 256, 0, st>>>(smp, ns, params, geo, key, cdf_raw);
-    S512_CHECK_LAUNCH();
-    k_knot_scan<<<1, 1024, 0, st>>>(cdf_raw, cdf);
+    k_knot_scan<<<1, 1024, 0, st>>>(ecdf_raw, ecdf);
     S512_CHECK_LAUNCH();
     if ((rc = ds.pinned2.reserve(4096, st))) return rc;
     char* h = (char*)ds.pinned2.ptr;
     S512_HIP(hipMemcpyAsync(h, geo, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-    S512_HIP(hipMemcpyAsync(h + 64, loss, sizeof(float), hipMemcpyDeviceToHost, st));
     S512_HIP(hipMemcpyAsync(h + 128, flag, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     if (stats) (void)hipEventRecord(e1, st);
+    S512_HIP(hipEventRecord(ds.lready, st));
     S512_HIP(hipStreamSynchronize(st));
     uint64_t g2[2];
     memcpy(g2, h, sizeof(g2));
-    float lossv;
-    memcpy(&lossv, h + 64, sizeof(float));
-    out->knots = LearnedModelDev{key, cdf};
-    out->degenerate = g2[1] <= g2[0];
-    out->loss = lossv;
-    out->params = params;
     uint32_t pm;
     memcpy(&pm, h + 128, sizeof(pm));
+    out->ecdf = LearnedModelDev{key, ecdf, smp, ns, (uint32_t)sizeof(K)};
+    out->knots = LearnedModelDev{key, cdf, smp, ns, (uint32_t)sizeof(K)};
+    out->params = params;
+    out->degenerate = g2[1] <= g2[0];
     out->point_mass = pm != 0;
     out->lf1 = (uint32_t)lf1;
+    out->training = false;
+    out->trained = ds.ltrained;
+    out->loss_host = (const float*)ds.lpinned.ptr;
     if (stats) {
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, e0, e1);
         stats->kernel_ms[S512_PHASE_TRAIN] += ms;
         stats->launches[S512_PHASE_TRAIN] += 1;
         stats->keys[S512_PHASE_TRAIN] += ns;
-        stats->model_loss = lossv;
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
     }
+    if (out->degenerate || out->point_mass) return S512_OK;
+    // the network: trained on the training stream while the caller runs the
+    // first level; its knots (same keys, its own CDF) serve the second level
+    hipStream_t ls = ds.lstream;
+    S512_HIP(hipStreamWaitEvent(ls, ds.lready, 0));
+    k_mlp_train<<<1, TRAIN_P, 0, ls>>>(params, u, t, M, prm.train_steps, prm.learn_rate, loss, nullptr);
+    S512_CHECK_LAUNCH();
+    k_knots<K><<<(LEARN_KNOTS + 1 + 255) / 256, 256, 0, ls>>>(smp, ns, params, geo, nullptr, cdf_raw);
+    S512_CHECK_LAUNCH();
+    k_knot_scan<<<1, 1024, 0, ls>>>(cdf_raw, cdf);
+    S512_CHECK_LAUNCH();
+    S512_HIP(hipMemcpyAsync(ds.lpinned.ptr, loss, sizeof(float), hipMemcpyDeviceToHost, ls));
+    S512_HIP(hipEventRecord(ds.ltrained, ls));
+    out->training = true;
     return S512_OK;
 }
 

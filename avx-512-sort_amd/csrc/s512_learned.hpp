@@ -11,10 +11,10 @@
 //        r(z) = min(max(z,0),1).  Squared weights make y non-decreasing in u.
 // Knots: the sorted sample is cut into LEARN_KNOTS pieces of equal mass; knot
 // k is the sample key of rank r_k = k (ns-1) / LEARN_KNOTS and its CDF value
-// is the network's prediction at that key, kept within half the sample mass
-// of the two adjacent pieces around the empirical value (the network smooths
-// within the sample's resolution, it cannot move a knot further), then made
-// non-decreasing by a running maximum.  Equal-mass knots put the model's
+// is the network's prediction at that key, kept within a quarter of a knot's
+// mass around the empirical value (the network smooths within the sample's
+// resolution, it cannot move a knot further; neighbouring bands do not
+// overlap), then made non-decreasing by a running maximum.  Equal-mass knots put the model's
 // resolution where the keys are, however skewed they are.
 // Splitters: the model's quantiles, i.e. the inverse of the piecewise-linear
 // CDF through the knots (k_model_splitters).  A level partitioned by the model
@@ -44,12 +44,63 @@ constexpr int LEARN_KNOTS = 8192;
 struct LearnedModelDev {
     const uint64_t* key;
     const float* cdf;
+    const void* smp = nullptr;  // the sorted sample (encoded keys) the knots are taken from
+    uint32_t ns = 0;            // its length
+    uint32_t kbytes = 8;        // bytes per sample key (4 or 8)
 };
 
 template <class K>
 __device__ __forceinline__ float learned_feature(K x, uint64_t kmin, float inv_range) {
     if ((uint64_t)x <= kmin) return 0.0f;
     return fminf((float)((uint64_t)x - kmin) * inv_range, 1.0f);
+}
+
+// Position (knot interval lo-1..lo and fraction f) where the piecewise-linear
+// CDF through the knots reaches q; the key there is key[lo-1] + f (key[lo] -
+// key[lo-1]).  lo = 0: below the first knot (f = 0); lo = LEARN_KNOTS + 1:
+// above the last.
+__device__ __forceinline__ void model_position(const float* cdf, double q, uint32_t* lo_out, double* f_out) {
+    if (q <= (double)cdf[0]) { *lo_out = 0; *f_out = 0.0; return; }
+    if (q >= (double)cdf[LEARN_KNOTS]) { *lo_out = LEARN_KNOTS + 1; *f_out = 0.0; return; }
+    uint32_t lo = 1, hi = LEARN_KNOTS;  // first knot with cdf >= q, in [1, LEARN_KNOTS]
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if ((double)cdf[mid] < q) lo = mid + 1;
+        else hi = mid;
+    }
+    const double c0 = cdf[lo - 1], c1 = cdf[lo];
+    const double f = c1 > c0 ? (q - c0) / (c1 - c0) : 1.0;
+    *lo_out = lo;
+    *f_out = f < 0.0 ? 0.0 : (f > 1.0 ? 1.0 : f);
+}
+
+// The CDF value another knot table over the same keys takes at that position.
+__device__ __forceinline__ double model_cdf_at(const float* cdf, uint32_t lo, double f) {
+    if (lo == 0) return cdf[0];
+    if (lo > LEARN_KNOTS) return cdf[LEARN_KNOTS];
+    return (double)cdf[lo - 1] + f * ((double)cdf[lo] - (double)cdf[lo - 1]);
+}
+
+// Encoded key at CDF value q: the sample key at the rank the model's
+// position maps to.  Knot k sits at sample rank k (ns - 1) / LEARN_KNOTS, so
+// a position (interval, fraction) interpolates the rank, not the key: the
+// quantiles are actual sample keys, spaced as the model says inside a knot's
+// interval, whatever the key encoding's spacing (the encoded space of doubles
+// is close to logarithmic).
+__device__ __forceinline__ uint64_t model_quantile_rank(const LearnedModelDev& m, double q) {
+    uint32_t lo;
+    double f;
+    model_position(m.cdf, q, &lo, &f);
+    const uint64_t last = (uint64_t)m.ns - 1;
+    uint64_t r;
+    if (lo == 0) r = 0;
+    else if (lo > LEARN_KNOTS) r = last;
+    else {
+        const uint64_t r0 = ((uint64_t)(lo - 1) * last) / LEARN_KNOTS, r1 = ((uint64_t)lo * last) / LEARN_KNOTS;
+        r = r0 + (uint64_t)(f * (double)(r1 - r0));
+        if (r > r1) r = r1;
+    }
+    return m.kbytes == 8 ? ((const uint64_t*)m.smp)[r] : (uint64_t)((const uint32_t*)m.smp)[r];
 }
 
 // Encoded key at CDF value q of the piecewise-linear model through the knots.

@@ -95,6 +95,9 @@ struct DeviceState {
     DevBuf ids;       // bucket of every key of a partition level (u16)
     HostBuf pinned;   // host<->device metadata staging
     HostBuf pinned2;
+    HostBuf lpinned;  // learned: the training loss, written by the training stream
+    hipStream_t lstream = nullptr;  // learned: training runs here, beside the first level
+    hipEvent_t lready = nullptr, ltrained = nullptr;
     std::vector<hipEvent_t> events;  // profiling event pool
 };
 

@@ -210,6 +210,22 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
             set_error("partition did not converge within %d levels", prm.max_levels);
             return S512_ERR_INTERNAL;
         }
+        if (level == 2 && lstate.pending) {
+            // the network trained beside the first level: its knots place the
+            // second level, unless training failed (then sampled splitters)
+            lstate.pending = false;
+            S512_HIP(hipEventSynchronize(lstate.trained));
+            const float loss = *lstate.loss_host;
+            if (stats) stats->model_loss = loss;
+            if (!std::isfinite(loss) || loss > lstate.max_loss) {
+                use_learned = false;
+                if (stats) {
+                    stats->learned_fallback = 1;
+                    stats->learned_levels = 1;
+                }
+            }
+            S512_HIP(hipStreamWaitEvent(st, lstate.trained, 0));
+        }
         const uint32_t m = (uint32_t)segs.size();
         uint64_t total = 0;
         for (auto& s : segs) total += s.size;
@@ -346,7 +362,8 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
         }
         if (any_learned) {
             pf.begin(S512_PHASE_SAMPLE);
-            k_model_splitters<K><<<m, 256, 0, st>>>(d_ps, d_cnt + 12, model, d_smp);
+            k_model_splitters<K><<<m, 256, 0, st>>>(d_ps, d_cnt + 12, level == 1 ? lstate.l1 : lstate.l2, lstate.l1,
+                                                     d_smp);
             S512_CHECK_LAUNCH();
             pf.end();
         }
@@ -514,13 +531,21 @@ s512_status sort_engine_locked(T* a, int32_t* va, size_t n, hipStream_t st, cons
     memset(&model, 0, sizeof(model));
     bool use_learned = false;
     uint32_t lf1 = 0;
+    // every return waits for a training still running on the training stream
+    // (it writes the model buffers the next call reuses)
+    struct TrainWait {
+        hipEvent_t ev = nullptr;
+        ~TrainWait() {
+            if (ev) (void)hipEventSynchronize(ev);
+        }
+    } train_wait;
     if constexpr (!KV && (std::is_same<T, double>::value || std::is_same<T, int32_t>::value)) {
         if (prm.learned) {
             if ((rc = build_learned_model<T>(a, n, st, ds, prm, &model, stats))) return rc;
-            // rejected outright: a point mass the network cannot split, or a
-            // network that did not train (loss not finite or above the limit)
-            const bool fits = std::isfinite(model.loss) && model.loss <= prm.max_loss;
-            const bool ok = !model.degenerate && !model.point_mass && fits;
+            if (model.training) train_wait.ev = model.trained;
+            // rejected outright: a point mass the network cannot split (a
+            // network that fails to train is caught before the second level)
+            const bool ok = !model.degenerate && !model.point_mass;
             if (!model.degenerate && !ok && stats) stats->learned_fallback = 1;
             use_learned = ok;
             lf1 = model.lf1;
@@ -536,7 +561,19 @@ s512_status sort_engine_locked(T* a, int32_t* va, size_t n, hipStream_t st, cons
     LevelState ls;
     ls.use_learned = use_learned;
     ls.lf1 = lf1;
-    return run_levels<T, KV>(a, va, b, vb, n, segs, st, prm, stats, pf, ds, model.knots, ls);
+    ls.l1 = model.ecdf;
+    ls.l2 = model.knots;
+    ls.trained = model.trained;
+    ls.loss_host = model.loss_host;
+    ls.max_loss = prm.max_loss;
+    ls.pending = use_learned && model.training;
+    rc = run_levels<T, KV>(a, va, b, vb, n, segs, st, prm, stats, pf, ds, model.knots, ls);
+    if (rc == S512_OK && stats && model.training && ls.pending) {
+        // the sort finished within one level: report the loss all the same
+        (void)hipEventSynchronize(model.trained);
+        stats->model_loss = *model.loss_host;
+    }
+    return rc;
 }
 
 template s512_status run_levels<int32_t, false>(int32_t*, int32_t*, int32_t*, int32_t*, size_t, std::vector<Seg>,

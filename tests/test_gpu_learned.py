@@ -126,9 +126,11 @@ def test_learned_vs_splitters_ab(s512, dev, capsys, kind):
 @pytest.mark.parametrize("kind", ["normal", "mixture"])
 def test_training_gain(s512, dev, capsys, kind):
     """What training buys: the CDF fit (full-batch MSE of the training points)
-    of the trained network against the network after one Adam step from its
-    initialisation (the piecewise-linear interpolation through 33 sample
-    quantiles), and the first-level bucket balance each one gives."""
+    of the trained network, whose knots place the second level's splitters,
+    against the network after one Adam step from its initialisation (the
+    piecewise-linear interpolation through 33 sample quantiles).  The first
+    level uses the sample's own CDF (the knots the network is kept within),
+    so its bucket balance is the same for every training length."""
     n = 16 << 20
     src = _fill(s512, dev, kind, n, seed=5)
     res = {}
@@ -142,4 +144,4 @@ def test_training_gain(s512, dev, capsys, kind):
                                            for k, v in res.items()))
     assert res[30][0] < 0.5 * res[1][0], res
     assert res[200][0] < res[30][0], res
-    assert res[30][1] <= res[1][1] * 1.05, res
+    assert res[30][1] == res[1][1] == res[200][1], res
