@@ -456,8 +456,9 @@ s512_status build_learned_model(const T* a, size_t n, hipStream_t st, DeviceStat
     }
     k_learn_sample<T><<<(ns + 255) / 256, 256, 0, st>>>(a, n, ns, prm.seed ^ 0x1EA2ull, smp);
     S512_CHECK_LAUNCH();
-    // the sample is sorted by this library's own engine (lock already held);
-    // leaves of 8192 keys: one partition level for a sample of a few million
+    // the sample is sorted by this library's own engine (lock already held),
+    // with leaves of up to 8192 keys: for a few million keys one partition
+    // level plus a short second one for the buckets above a tile
     EngineParams sp = default_params(sizeof(K) == 8);
     sp.target_leaf = 8192;
     if ((rc = sort_engine_locked<K, false>(smp, nullptr, ns, st, sp, nullptr, false, ds))) return rc;
