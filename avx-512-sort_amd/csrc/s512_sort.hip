@@ -25,7 +25,8 @@ namespace s512 {
 
 // ---------------------------------------------------------------- tile classes
 static const uint32_t kCaps32[] = {256, 512, 1024, 2048, 4096, 8192, 16384};
-static const uint32_t kCaps64[] = {256, 512, 1024, 2048, 4096, 8192};
+static const uint32_t kCaps64[] = {256, 512, 1024, 2048, 4096, 8192, 16384};  // 8-byte keys
+static const uint32_t kCapsKV[] = {256, 512, 1024, 2048, 4096, 8192};         // key/value pairs (and segmented 8-byte keys)
 
 // Tile shapes (threads P x keys per thread E) per class and plan.  Plan 0
 // keeps 8 keys per thread up to 2048-key tiles; plan 1 uses 16; plan 2 uses 32
@@ -76,6 +77,9 @@ static void launch_tile_keys_pol(int cls, int plan, const TileSeg* segs, const u
             case 16 + 3: S512_TILE(64, 32); break;
             case 16 + 4: S512_TILE(128, 32); break;
             case 16 + 5: S512_TILE(256, 32); break;
+            case 6:
+            case 8 + 6: S512_TILE(1024, 16); break;  // 128 KiB of LDS
+            case 16 + 6: S512_TILE(512, 32); break;
             default: S512_TILE(512, 16); break;  // class 5 of plans 0, 1
         }
     }
@@ -132,9 +136,9 @@ static void launch_tile_kv64(int cls, const TileSeg* segs, const uint32_t* dcoun
 
 template <class T, bool KV>
 struct TileCfg {
-    static constexpr bool WIDE = KV || sizeof(T) == 8;
-    static const uint32_t* caps() { return WIDE ? kCaps64 : kCaps32; }
-    static int nclass() { return WIDE ? 6 : 7; }
+    // key/value tiles stop at 8192 pairs (LDS); plain keys of either width at 16384
+    static const uint32_t* caps() { return KV ? kCapsKV : sizeof(T) == 8 ? kCaps64 : kCaps32; }
+    static int nclass() { return KV ? 6 : 7; }
     static uint32_t maxcap() { return caps()[nclass() - 1]; }
     static int cls_of(uint64_t n) {
         for (int c = 0; c < nclass(); ++c)
@@ -160,9 +164,16 @@ static inline int ceil_log2(uint64_t x) {
     return l;
 }
 
-// Tile classes of plain keys, for callers outside this file.
+// Tile classes of plain keys for the segmented sort.  For 8-byte keys they
+// stop at 8192: a segment of 8193..16384 keys partitions there, which measured
+// faster than the 16384-key tile (whose registers spill; the sort engine uses
+// that tile only for the rare bucket just above 8192 keys).
 template <class T>
 const uint32_t* tile_caps(int* ncls) {
+    if (sizeof(T) == 8) {
+        *ncls = 6;
+        return kCapsKV;
+    }
     *ncls = TileCfg<T, false>::nclass();
     return TileCfg<T, false>::caps();
 }

@@ -33,6 +33,9 @@ void launch_tile_f64_fast(int cls, int plan, const TileSeg* segs, const uint32_t
         case 16 + 3: S512_TF(64, 32); break;
         case 16 + 4: S512_TF(128, 32); break;
         case 16 + 5: S512_TF(256, 32); break;
+        case 6:
+        case 8 + 6: S512_TF(1024, 16); break;
+        case 16 + 6: S512_TF(512, 32); break;
         default: S512_TF(512, 16); break;
     }
 #undef S512_TF

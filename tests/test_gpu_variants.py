@@ -123,3 +123,23 @@ def test_tuned_defaults_kv_and_segmented(s512, orc, dev):
     got = to_host(tk)
     for i in range(len(off) - 1):
         assert np.array_equal(got[off[i]:off[i + 1]], np.sort(keys[off[i]:off[i + 1]])), i
+
+
+@pytest.mark.parametrize("variant", range(1, 4))
+@pytest.mark.parametrize("dtype", [np.float64, np.int64])
+def test_tile_16k_class_after_partition(s512, orc, dev, dtype, variant):
+    """Buckets of 8193..16384 eight-byte keys left by a partition level go to
+    the 16384-key tile: fan-out 2 halves 100,000 keys into buckets of about
+    50,000, 25,000 and then 12,500 keys.  fp64 without NaN or -0.0 sorts them
+    with min/max leaves, int64 with encoded compares; every tile shape."""
+    n = 100_000
+    if dtype == np.float64:
+        a = orc.gen_f64("normal", n, 13)
+    else:
+        a = np.random.default_rng(13).integers(-(1 << 63), (1 << 63) - 1, n, dtype=np.int64)
+    got, st = _run(s512, dev, a, tile_variant=variant, max_fanout_log2=1)
+    assert st.levels >= 3, st.levels
+    if dtype == np.float64:
+        assert_keys_equal(got, orc.closed_sort(a.copy()), f"tile_variant={variant}")
+    else:
+        assert np.array_equal(got, np.sort(a)), f"tile_variant={variant}"
