@@ -838,15 +838,18 @@ inline void launch_count(int shape, uint32_t grid, hipStream_t st, const PartSeg
 // ---------------------------------------------------------------- column scan
 // hist[chunk][bucket] of every segment -> prefix over earlier chunks of the
 // same bucket (in place); bsize[b] = bucket size.  One workgroup per
-// (segment, slice of 64 buckets): lane = bucket, wave = a quarter of the
-// chunks; loads are batched so a column of ~1000 chunks is not a chain of
-// dependent round trips.
+// (segment, slice of 64 buckets): lane = bucket, wave = 1/NG of the chunks;
+// loads are batched so a column of thousands of chunks is not a chain of
+// dependent round trips.  The host launches NG = 16 when a segment has many
+// chunks (the root level: thousands of chunks in one segment, only 16
+// workgroups in all) and NG = 4 otherwise.
 constexpr int CS_SLICE = 64;
 constexpr int CS_NSLICE = BMAX / CS_SLICE;
-constexpr int CS_BATCH = 8;
-[[maybe_unused]] static __global__ __launch_bounds__(256) void k_colscan(const PartSeg* ps, const uint32_t* nseg,
-                                                                        uint32_t* hist, uint64_t* bsize) {
-    __shared__ uint32_t part[4][CS_SLICE];
+constexpr int CS_BATCH = 16;
+template <int NG>
+__global__ __launch_bounds__(NG * 64) void k_colscan(const PartSeg* ps, const uint32_t* nseg, uint32_t* hist,
+                                                     uint64_t* bsize) {
+    __shared__ uint32_t part[NG][CS_SLICE];
     const uint32_t lane = threadIdx.x & 63u, grp = threadIdx.x >> 6;
     const uint32_t nwork = *nseg * CS_NSLICE;
     for (uint32_t w = blockIdx.x; w < nwork; w += gridDim.x) {
@@ -857,7 +860,7 @@ constexpr int CS_BATCH = 8;
         const uint32_t bkt = b0 + lane;
         const bool on = bkt < B;
         const uint32_t nc = s.nchunks;
-        const uint32_t c0 = (uint32_t)((uint64_t)nc * grp / 4), c1 = (uint32_t)((uint64_t)nc * (grp + 1) / 4);
+        const uint32_t c0 = (uint32_t)((uint64_t)nc * grp / NG), c1 = (uint32_t)((uint64_t)nc * (grp + 1) / NG);
         uint32_t* h = hist + s.hist_base + bkt;
         uint32_t sum = 0;
         for (uint32_t cb = c0; cb < c1; cb += CS_BATCH) {
@@ -871,7 +874,7 @@ constexpr int CS_BATCH = 8;
         __syncthreads();
         uint32_t run = 0, total = 0;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < NG; ++q) {
             const uint32_t p = part[q][lane];
             run += q < (int)grp ? p : 0u;
             total += p;

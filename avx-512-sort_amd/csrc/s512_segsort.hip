@@ -292,7 +292,7 @@ static s512_status seg_enqueue(T* a, T* b, const uint64_t* off, size_t nseg, con
     launch_count<T>(prm.count_shape, grid_of(nchunk_cap, 8192), st, w.ps, w.cr, nchunk, a, b, (K*)w.trees, w.guides,
                     w.hist, w.counters + CNT_SPECIAL, nullptr);
     S512_CHECK_LAUNCH();
-    k_colscan<<<grid_of((uint64_t)nbig_cap * CS_NSLICE, 8192), 256, 0, st>>>(w.ps, nbig, w.hist, w.bsize);
+    k_colscan<4><<<grid_of((uint64_t)nbig_cap * CS_NSLICE, 8192), 256, 0, st>>>(w.ps, nbig, w.hist, w.bsize);
     S512_CHECK_LAUNCH();
     k_bucket_scan<<<grid_of(nbig_cap, 2048), BMAX, 0, st>>>(w.ps, nbig, w.bsize, w.bstart);
     S512_CHECK_LAUNCH();
