@@ -845,10 +845,10 @@ inline void launch_count(int shape, uint32_t grid, hipStream_t st, const PartSeg
 // workgroups in all) and NG = 4 otherwise.
 constexpr int CS_SLICE = 64;
 constexpr int CS_NSLICE = BMAX / CS_SLICE;
-constexpr int CS_BATCH = 16;
 template <int NG>
 __global__ __launch_bounds__(NG * 64) void k_colscan(const PartSeg* ps, const uint32_t* nseg, uint32_t* hist,
                                                      uint64_t* bsize) {
+    constexpr int CS_BATCH = NG >= 16 ? 16 : 8;  // loads in flight per lane
     __shared__ uint32_t part[NG][CS_SLICE];
     const uint32_t lane = threadIdx.x & 63u, grp = threadIdx.x >> 6;
     const uint32_t nwork = *nseg * CS_NSLICE;
