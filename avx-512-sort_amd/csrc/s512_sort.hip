@@ -317,7 +317,9 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
         sc.take<Seg>(sumB);
         if ((rc = ds.arena.reserve(sc.off, st))) return rc;
         if ((rc = ds.pinned.reserve(up_bytes, st))) return rc;
-        if ((rc = ds.pinned2.reserve(std::max<size_t>(sumB * sizeof(Seg), 4096), st))) return rc;
+        // pinned2: the level's counts (first 4 KiB), then the next level's segments
+        const uint64_t pre_next = std::min<uint64_t>(sumB, 4096);  // segments read with the counts
+        if ((rc = ds.pinned2.reserve(4096 + std::max<size_t>(sumB, 1) * sizeof(Seg), st))) return rc;
 
         Arena ar{(char*)ds.arena.ptr, 0}, hp{(char*)ds.pinned.ptr, 0};
         PartSeg* d_ps = ar.take<PartSeg>(m);
@@ -437,7 +439,11 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
         keys_ph[S512_PHASE_COUNT] += total;
         keys_ph[S512_PHASE_SCATTER] += total;
         uint32_t* h_out = (uint32_t*)ds.pinned2.ptr;
+        Seg* h_next = (Seg*)((char*)ds.pinned2.ptr + 4096);
         S512_HIP(hipMemcpyAsync(h_out, d_cnt, 32 * sizeof(uint32_t) + 256, hipMemcpyDeviceToHost, st));
+        // the first segments of the next level come back with the counts: one
+        // host round trip per level instead of two
+        S512_HIP(hipMemcpyAsync(h_next, pl.next, pre_next * sizeof(Seg), hipMemcpyDeviceToHost, st));
         S512_HIP(hipStreamSynchronize(st));
         uint32_t counts[32];
         uint64_t counts64[16];
@@ -473,9 +479,11 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
                 set_error("plan produced %u segments for %llu buckets", nn, (unsigned long long)sumB);
                 return S512_ERR_INTERNAL;
             }
-            Seg* h_next = (Seg*)ds.pinned2.ptr;
-            S512_HIP(hipMemcpyAsync(h_next, pl.next, nn * sizeof(Seg), hipMemcpyDeviceToHost, st));
-            S512_HIP(hipStreamSynchronize(st));
+            if (nn > pre_next) {
+                S512_HIP(hipMemcpyAsync(h_next + pre_next, pl.next + pre_next, (nn - pre_next) * sizeof(Seg),
+                                        hipMemcpyDeviceToHost, st));
+                S512_HIP(hipStreamSynchronize(st));
+            }
             segs.assign(h_next, h_next + nn);
         }
     }
