@@ -782,10 +782,13 @@ __global__ __launch_bounds__(CNT_P) void k_count(const PartSeg* ps, const ChunkR
                 for (int u = 0; u < NB; ++u) bk[u0 + u] = kb[u];
             }
             wave_hist_add_many<CNT_U>(lh, dummy, bk);
+            if (special) {
 #pragma unroll
-            for (int u = 0; u < CNT_U; ++u) {
-                odd |= nan_or_negzero(x[u]);
-                if (oid) oid[base + (uint64_t)u * CNT_P + t] = (uint16_t)bk[u];
+                for (int u = 0; u < CNT_U; ++u) odd |= nan_or_negzero(x[u]);
+            }
+            if (oid) {
+#pragma unroll
+                for (int u = 0; u < CNT_U; ++u) oid[base + (uint64_t)u * CNT_P + t] = (uint16_t)bk[u];
             }
         } else {
 #pragma unroll
@@ -793,7 +796,7 @@ __global__ __launch_bounds__(CNT_P) void k_count(const PartSeg* ps, const ChunkR
                 const uint64_t i = base + (uint64_t)u * CNT_P + t;
                 if (i < hi) {
                     const uint32_t bk = classify(lsrt, lguide, g, x[u]);
-                    odd |= nan_or_negzero(x[u]);
+                    if (special) odd |= nan_or_negzero(x[u]);
                     atomicAdd(&lh[bk], 1u);
                     if (oid) oid[i] = (uint16_t)bk;
                 }
@@ -809,7 +812,7 @@ __global__ __launch_bounds__(CNT_P) void k_count(const PartSeg* ps, const ChunkR
         batch(base, xb, xa);
         base += STEP;
     }
-    if (__any(odd) && (t & 63u) == 0) atomicOr(special, 1u);
+    if (special && __any(odd) && (t & 63u) == 0) atomicOr(special, 1u);
     __syncthreads();
     uint32_t* h = hist + s.hist_base + (uint64_t)c.idx * B;
     for (uint32_t j = t; j < B; j += CNT_P) h[j] = lh[j];

@@ -393,8 +393,9 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
         S512_CHECK_LAUNCH();
         pf.end();
         pf.begin(S512_PHASE_COUNT);
-        launch_count<T>(prm.count_shape, tc, st, d_ps, d_cr, d_cnt + 13, a, b, d_tree, d_guide, d_hist, d_cnt + 11,
-                        d_ids);
+        // NaN / -0.0 keys are looked for in the first level only (it sees every key)
+        launch_count<T>(prm.count_shape, tc, st, d_ps, d_cr, d_cnt + 13, a, b, d_tree, d_guide, d_hist,
+                        level == 1 ? d_cnt + 11 : nullptr, d_ids);
         S512_CHECK_LAUNCH();
         pf.end();
         pf.begin(S512_PHASE_SCAN);
