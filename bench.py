@@ -55,14 +55,15 @@ KERNEL_OF_PHASE = {"count": "k_count", "scatter": "k_scatter", "tile": "k_tile_s
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    # five timed steps after two warm-ups (each step sorts every distribution once)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=268_435_456)
     ap.add_argument("--dists", default=",".join(DISTS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 --pmc child runs")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-splitters comparison")
-    ap.add_argument("--cpu-sample", type=int, default=1 << 25)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 28)  # the full 268,435,456-key uniform input
     ap.add_argument("--workload", default="plain", choices=list(WORKLOADS),
                     help="plain: the 256M fp64 headline (default); the others time the secondary workloads "
                          "(256M int32, host-pointer end to end, int64 keys and fp64 argsort, top-k, key/value Zipf, "
