@@ -143,3 +143,23 @@ def test_tile_16k_class_after_partition(s512, orc, dev, dtype, variant):
         assert_keys_equal(got, orc.closed_sort(a.copy()), f"tile_variant={variant}")
     else:
         assert np.array_equal(got, np.sort(a)), f"tile_variant={variant}"
+
+
+def test_levels_with_thousands_of_next_segments(s512, dev):
+    """Fan-out 2 on 268,435,456 int32 keys: about 14 partition levels, with
+    8,192 segments handed to the last one (more than the 4,096 that come back
+    with a level's counts, so the rest are read separately).  Sorted and the
+    same multiset of keys."""
+    import torch
+    n = 1 << 28
+    t = torch.empty(n, dtype=torch.int32, device=dev)
+    s512.fill(t, "uniform", seed=21)
+    x = t.to(torch.int64) * -7046029254386353131
+    h0 = int((x ^ (x >> 29)).sum().item())
+    o = s512.Options()
+    o.max_fanout_log2 = 1
+    st = s512.sort_ex(t, o)
+    assert st.levels >= 13, st.levels
+    assert bool(torch.all(t[1:] >= t[:-1]).item())
+    x = t.to(torch.int64) * -7046029254386353131
+    assert int((x ^ (x >> 29)).sum().item()) == h0
