@@ -377,7 +377,15 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
             S512_CHECK_LAUNCH();
             pf.end();
             pf.begin(S512_PHASE_SAMPLE_SORT);
-            k_tile_sort<K, 512, 16><<<nsplit, 512, 0, st>>>(d_sl, d_cnt + 10, d_smp, d_smp);
+            // the smallest tile that holds every segment's sample: a level of
+            // many small segments (fan-out below 512) samples 16F < 8192 keys each
+            uint32_t max_ns = 0;
+            for (auto& p : ps)
+                if (!p.mode) max_ns = std::max(max_ns, p.nsample);
+            if (max_ns <= 1024) k_tile_sort<K, 64, 16><<<nsplit, 64, 0, st>>>(d_sl, d_cnt + 10, d_smp, d_smp);
+            else if (max_ns <= 2048) k_tile_sort<K, 128, 16><<<nsplit, 128, 0, st>>>(d_sl, d_cnt + 10, d_smp, d_smp);
+            else if (max_ns <= 4096) k_tile_sort<K, 256, 16><<<nsplit, 256, 0, st>>>(d_sl, d_cnt + 10, d_smp, d_smp);
+            else k_tile_sort<K, 512, 16><<<nsplit, 512, 0, st>>>(d_sl, d_cnt + 10, d_smp, d_smp);
             S512_CHECK_LAUNCH();
             pf.end();
         }
