@@ -294,6 +294,45 @@ __global__ __launch_bounds__(P) void k_tile_sort(const TileSeg* segs, const uint
     }
 }
 
+// Sample sorts of many segments whose sample sizes differ (the segmented
+// sort's large segments: 16 F samples for fan-outs F from 2 to 512).  One
+// launch per network size P*E sorts the listed segments with lo < size <=
+// P*E and skips the others, so a sample runs the smallest network that holds
+// it instead of being padded to the largest.  (One kernel choosing among the
+// four networks per segment needs their registers at once, and spills.)
+template <class T, int P, int E, class Pol = DefaultOrder<T>>
+__global__ __launch_bounds__(P) void k_tile_sort_range(const TileSeg* segs, const uint32_t* count, T* a, const T* b,
+                                                       uint32_t lo) {
+    using IO = KeyIO<T, Pol>;
+    using K = typename Pol::key_t;
+    using BB = BlockBitonic<Pol, P, E>;
+    __shared__ K lds[BB::N];
+    const uint32_t cnt = *count;
+    const uint32_t t = threadIdx.x;
+    for (uint32_t s = blockIdx.x; s < cnt; s += gridDim.x) {
+        const TileSeg sg = segs[s];
+        if (sg.size <= lo || sg.size > (uint32_t)BB::N) continue;  // uniform: the whole workgroup skips
+        const T* src = (sg.buf ? b : a) + sg.begin;
+        T* dst = a + sg.begin;
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const uint32_t i = m * P + t;
+            const K x = IO::load(src[i < sg.size ? i : sg.size - 1]);
+            lds[BB::phys(i)] = i < sg.size ? x : IO::pad();
+        }
+        K v[E];
+        BB::from_striped(v, lds, t);
+        BB::sort(v, lds, t);
+        BB::to_striped(v, lds, t);
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const uint32_t i = m * P + t;
+            if (i < sg.size) dst[i] = IO::store(lds[BB::phys(i)]);
+        }
+        __syncthreads();
+    }
+}
+
 // One wave per segment (size <= 64*E), four waves per workgroup.
 template <class T, int E, class Pol = DefaultOrder<T>>
 __global__ __launch_bounds__(256) void k_wave_sort(const TileSeg* segs, const uint32_t* count, T* a, const T* b) {

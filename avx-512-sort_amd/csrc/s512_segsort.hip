@@ -284,7 +284,18 @@ static s512_status seg_enqueue(T* a, T* b, const uint64_t* off, size_t nseg, con
     const uint32_t* nchunk = w.counters + CNT_NCHUNK;
     k_sample<T><<<grid_of(nbig_cap, 2048), 256, 0, st>>>(w.ps, nbig, a, b, (K*)w.samples, SEG_SMAX, prm.seed);
     S512_CHECK_LAUNCH();
-    k_tile_sort<K, 512, 16><<<grid_of(nbig_cap, 2048), 512, 0, st>>>(w.sl, nbig, (K*)w.samples, (K*)w.samples);
+    // each sample by the smallest of four networks (1024..8192 keys) that holds it
+    {
+        const uint32_t g = grid_of(nbig_cap, 2048);
+        K* smp = (K*)w.samples;
+        k_tile_sort_range<K, 512, 2><<<g, 512, 0, st>>>(w.sl, nbig, smp, smp, 0u);
+        S512_CHECK_LAUNCH();
+        k_tile_sort_range<K, 512, 4><<<g, 512, 0, st>>>(w.sl, nbig, smp, smp, 1024u);
+        S512_CHECK_LAUNCH();
+        k_tile_sort_range<K, 512, 8><<<g, 512, 0, st>>>(w.sl, nbig, smp, smp, 2048u);
+        S512_CHECK_LAUNCH();
+        k_tile_sort_range<K, 512, 16><<<g, 512, 0, st>>>(w.sl, nbig, smp, smp, 4096u);
+    }
     S512_CHECK_LAUNCH();
     k_build_tree<K><<<grid_of(nbig_cap, 2048), FMAX, 0, st>>>(w.ps, nbig, (K*)w.samples, SEG_SMAX, (K*)w.trees,
                                                               w.guides);
