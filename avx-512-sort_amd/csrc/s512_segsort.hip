@@ -22,10 +22,12 @@
 // A bucket still larger than the largest tile lands on the "next" list; the
 // host finishes those with the multi-level engine after the sequence.
 #include "s512_engine_internal.hpp"
+#include "s512_tile_f64.hpp"
 #include "s512_hostio.hpp"
 #include "../../include/s512.h"
 #include <algorithm>
 #include <string.h>
+#include <type_traits>
 #include <vector>
 
 namespace s512 {
@@ -315,8 +317,12 @@ static s512_status seg_enqueue(T* a, T* b, const uint64_t* off, size_t nseg, con
     k_copy_back<T, false><<<2048, 256, 0, st>>>(w.copy, w.counters + CNT_COPY, a, b, nullptr, nullptr);
     S512_CHECK_LAUNCH();
     for (uint32_t k = 0; k < w.cls.pl.nclass; ++k) {
-        launch_tile_fixed<T>((int)k, prm.tile_plan, w.tile[k], w.counters + k, grid_of(w.cls.list_cap[k], 32768), a, b,
-                             st);
+        if constexpr (std::is_same<T, double>::value)
+            launch_tile_f64_auto((int)k, prm.tile_plan, w.tile[k], w.counters + k, grid_of(w.cls.list_cap[k], 32768),
+                                 a, b, st);
+        else
+            launch_tile_fixed<T>((int)k, prm.tile_plan, w.tile[k], w.counters + k,
+                                 grid_of(w.cls.list_cap[k], 32768), a, b, st);
         S512_CHECK_LAUNCH();
     }
     return S512_OK;
