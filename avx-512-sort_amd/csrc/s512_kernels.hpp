@@ -1315,8 +1315,11 @@ __global__ __launch_bounds__(256) void k_copy_back(const TileSeg* pieces, const 
     const uint32_t cnt = *count;
     for (uint32_t p = blockIdx.x; p < cnt; p += gridDim.x) {
         const TileSeg sg = pieces[p];
+        // a piece is part of an equality bucket or a single key: all its keys
+        // have the bits of its first key, so they are written, not copied
+        const T key = b[sg.begin];
         for (uint32_t i = threadIdx.x; i < sg.size; i += blockDim.x) {
-            a[sg.begin + i] = b[sg.begin + i];
+            a[sg.begin + i] = key;
             if constexpr (KV) va[sg.begin + i] = vb[sg.begin + i];
         }
     }
