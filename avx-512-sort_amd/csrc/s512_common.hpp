@@ -95,6 +95,11 @@ struct PartSeg {
     uint32_t lbase;
     uint32_t flags;      // classifier flags (GUIDE_SUBTABLES, GUIDE_EQBUCKETS), set on the device
     uint64_t sample_base;  // offset of this segment's samples in the sample buffer
+    // one-pass levels only (s512_onepass.hpp)
+    uint64_t dst;        // exact mode: position of the segment's first key in the destination;
+                         // gapped mode: first slot of the segment's pair regions
+    uint64_t fin;        // position of the segment's first key in sorted order (caller's array)
+    uint64_t st_base;    // first look-back status word of the segment ([nchunks][2F])
 };
 
 struct ChunkRef {

@@ -25,6 +25,9 @@ struct EngineParams {
     bool ids;                // count stores each key's bucket; scatter reads it
     int count_shape;         // k_count shape (0 .. COUNT_SHAPES-1)
     int tile_plan;           // tile sort shapes (0 .. TILE_PLANS-1)
+    int engine;              // S512_ENGINE_*: two-pass levels (auto) or one-pass levels
+    uint64_t onepass_min;    // one-pass: arrays of at least this many keys take the one-pass levels
+    float region_z;          // one-pass root level: region margin in standard deviations of its estimate
 };
 
 EngineParams default_params(bool wide_keys);

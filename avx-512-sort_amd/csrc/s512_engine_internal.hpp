@@ -34,6 +34,23 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
 template <class T>
 const uint32_t* tile_caps(int* ncls);
 
+// The engine's tile classes and launches (keys, or int32 key/value pairs):
+// sort every listed segment of class cls from (buf ? b : a) into a.
+template <class T, bool KV>
+const uint32_t* tile_class_caps(int* ncls);
+template <class T, bool KV>
+void launch_tile_list(int cls, int plan, const TileSeg* segs, const uint32_t* dcount, uint32_t count, T* a,
+                      int32_t* va, const T* b, const int32_t* vb, hipStream_t st, bool f64_fast);
+
+// One-pass levels (s512_onepass.hip): the root level and the second level
+// each read and write every key once; segments that need more levels are
+// finished by run_levels.  *done = false (with nothing changed) when the
+// root level cannot be done that way (a capacity region overflowed); the
+// caller then runs the multi-level engine on the whole array.
+template <class T, bool KV>
+s512_status onepass_engine(T* a, int32_t* va, size_t n, hipStream_t st, const EngineParams& prm, s512_stats* stats,
+                           Profiler& pf, DeviceState& ds, bool* done);
+
 // Launch the tile sort of class cls over the device-counted list segs with a
 // grid sized for grid_items segments (the kernels loop over the true count).
 template <class T>

@@ -164,6 +164,29 @@ static inline int ceil_log2(uint64_t x) {
     return l;
 }
 
+// The tile launches and classes of the engine, for the one-pass levels.
+template <class T, bool KV>
+void launch_tile_list(int cls, int plan, const TileSeg* segs, const uint32_t* dcount, uint32_t count, T* a,
+                      int32_t* va, const T* b, const int32_t* vb, hipStream_t st, bool f64_fast) {
+    launch_tile<T, KV>(cls, plan, segs, dcount, count, a, va, b, vb, st, f64_fast);
+}
+template <class T, bool KV>
+const uint32_t* tile_class_caps(int* ncls) {
+    *ncls = TileCfg<T, KV>::nclass();
+    return TileCfg<T, KV>::caps();
+}
+#define S512_TILE_LIST(T, KV)                                                                                    \
+    template void launch_tile_list<T, KV>(int, int, const TileSeg*, const uint32_t*, uint32_t, T*, int32_t*,     \
+                                          const T*, const int32_t*, hipStream_t, bool);                          \
+    template const uint32_t* tile_class_caps<T, KV>(int*);
+S512_TILE_LIST(int32_t, false)
+S512_TILE_LIST(int32_t, true)
+S512_TILE_LIST(double, false)
+S512_TILE_LIST(int64_t, false)
+S512_TILE_LIST(uint64_t, false)
+S512_TILE_LIST(uint32_t, false)
+#undef S512_TILE_LIST
+
 // Tile classes of plain keys for the segmented sort.  For 8-byte keys they
 // stop at 8192: a segment of 8193..16384 keys partitions there, which measured
 // faster than the 16384-key tile (whose registers spill; the sort engine uses
@@ -588,6 +611,17 @@ s512_status sort_engine_locked(T* a, int32_t* va, size_t n, hipStream_t st, cons
             lf1 = model.lf1;
         }
     }
+    // ---- one-pass levels, when asked for (not for the learned model's levels,
+    // nor 64-bit keys with values); they hand back with nothing changed when
+    // the root level's regions overflow, and the two-pass levels below run
+    if constexpr (!(KV && sizeof(T) == 8)) {
+        const bool want = prm.engine == S512_ENGINE_ONE_PASS && (uint64_t)n >= prm.onepass_min;
+        if (want && !use_learned) {
+            bool done = false;
+            if ((rc = onepass_engine<T, KV>(a, va, n, st, prm, stats, pf, ds, &done))) return rc;
+            if (done) return S512_OK;
+        }
+    }
     // ---- scratch arrays (same size as the input)
     const size_t dbytes = n * sizeof(T) + (KV ? n * sizeof(int32_t) : 0);
     if ((rc = ds.data.reserve(dbytes, st))) return rc;
@@ -605,6 +639,7 @@ s512_status sort_engine_locked(T* a, int32_t* va, size_t n, hipStream_t st, cons
     ls.max_loss = prm.max_loss;
     ls.pending = use_learned && model.training;
     rc = run_levels<T, KV>(a, va, b, vb, n, segs, st, prm, stats, pf, ds, model.knots, ls);
+    if (stats) stats->engine = S512_ENGINE_TWO_PASS;
     if (rc == S512_OK && stats && model.training && ls.pending) {
         // the sort finished within one level: report the loss all the same
         (void)hipEventSynchronize(model.trained);
@@ -651,6 +686,9 @@ EngineParams default_params(bool wide) {
     p.ids = true;
     p.count_shape = 2;  // 256 threads x 8 keys, 8 classified together (fastest measured)
     p.tile_plan = 2;    // 32 keys per thread above 1024-key tiles (fastest measured)
+    p.engine = S512_ENGINE_AUTO;
+    p.onepass_min = 1;  // any array with a partition level
+    p.region_z = 5.0f;  // P(a region overflows) ~ 3e-7 per (ordinary, equality) pair
     return p;
 }
 
@@ -672,6 +710,8 @@ EngineParams params_from(const s512_options* opt, bool wide) {
     }
     if (opt->count_variant) p.count_shape = (int)opt->count_variant - 1;
     if (opt->tile_variant) p.tile_plan = (int)opt->tile_variant - 1;
+    p.engine = (int)opt->engine;
+    if (opt->region_slack) p.region_z = (float)opt->region_slack / 10.0f;
     return p;
 }
 
@@ -734,6 +774,10 @@ static s512_status check_opts(const s512_options* opt) {
     }
     if (opt && (opt->sample_permille > 1000 || opt->train_steps > 100000)) {
         set_error("sample_permille (<= 1000) or train_steps (<= 100000) out of range");
+        return S512_ERR_INVALID;
+    }
+    if (opt && opt->engine > S512_ENGINE_ONE_PASS) {
+        set_error("engine %u out of range (max %d)", opt->engine, S512_ENGINE_ONE_PASS);
         return S512_ERR_INVALID;
     }
     if (opt && opt->tile_variant > TILE_PLANS) {

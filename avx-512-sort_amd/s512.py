@@ -135,7 +135,11 @@ class Options(ctypes.Structure):
                 ("classifier", ctypes.c_uint32), ("sample_permille", ctypes.c_uint32),
                 ("train_steps", ctypes.c_uint32), ("imbalance_pct", ctypes.c_uint32),
                 ("scatter_variant", ctypes.c_uint32), ("count_variant", ctypes.c_uint32),
-                ("tile_variant", ctypes.c_uint32)]
+                ("tile_variant", ctypes.c_uint32), ("engine", ctypes.c_uint32),
+                ("region_slack", ctypes.c_uint32)]
+
+
+ENGINE_AUTO, ENGINE_TWO_PASS, ENGINE_ONE_PASS = 0, 1, 2
 
 
 SPLITTERS, LEARNED = 0, 1
@@ -147,10 +151,11 @@ class Stats(ctypes.Structure):
                 ("launches", ctypes.c_uint32 * NPHASE), ("keys", ctypes.c_uint64 * NPHASE),
                 ("classifier", ctypes.c_int32), ("learned_fallback", ctypes.c_int32),
                 ("learned_levels", ctypes.c_int32), ("learned_max_bucket_ratio", ctypes.c_float),
-                ("model_loss", ctypes.c_float)]
+                ("model_loss", ctypes.c_float), ("engine", ctypes.c_int32), ("region_overflow", ctypes.c_int32)]
 
     def as_dict(self):
         return {"levels": self.levels, "tiles": self.tiles, "classifier": self.classifier,
+                "engine": self.engine, "region_overflow": self.region_overflow,
                 "learned_fallback": self.learned_fallback, "learned_levels": self.learned_levels,
                 "learned_max_bucket_ratio": self.learned_max_bucket_ratio, "model_loss": self.model_loss,
                 "phases": {PHASES[i]: {"ms": self.kernel_ms[i], "launches": self.launches[i], "keys": self.keys[i]}

@@ -212,7 +212,23 @@ typedef struct s512_options {
     uint32_t tile_variant;    /* 0: default (plan 2); else 1 + tile plan 0..2 = keys per
                                  thread of the bitonic tile sorts: 8, 16 or 32 (up to 16
                                  on 64-bit tiles of plan 0)                             */
+    uint32_t engine;          /* S512_ENGINE_*: how partition levels read the keys      */
+    uint32_t region_slack;    /* one-pass root level: margin of each capacity region over
+                                 its sampled estimate, in tenths of the estimate's standard
+                                 deviation (0: 50).  Smaller saves scratch memory and
+                                 overflows more often; an overflow redoes the level with
+                                 two passes                                            */
 } s512_options;
+
+enum {
+    S512_ENGINE_AUTO = 0,     /* the two-pass levels (measured faster on MI355X)          */
+    S512_ENGINE_TWO_PASS = 1, /* every level counts (one read), then scatters (read+write) */
+    S512_ENGINE_ONE_PASS = 2  /* the first two levels read and write every key once: the
+                                 root level into sample-sized regions of the scratch
+                                 array, the second level straight to final positions; the
+                                 root level falls back to two passes when a region
+                                 overflows (the input is unchanged by then)            */
+};
 
 typedef struct s512_stats {
     int32_t levels;                     /* partition levels run              */
@@ -225,6 +241,10 @@ typedef struct s512_stats {
     int32_t learned_levels;             /* levels partitioned by the model    */
     float learned_max_bucket_ratio;     /* largest first-level bucket / mean  */
     float model_loss;                   /* final training loss (MSE on CDF)   */
+    int32_t engine;                     /* S512_ENGINE_ONE_PASS or _TWO_PASS: how the
+                                           partition levels ran (0: no level) */
+    int32_t region_overflow;            /* 1: a one-pass root region overflowed and
+                                           the sort ran with two passes        */
 } s512_stats;
 
 /* Sort512::Sort with options and statistics (device pointers). */
