@@ -1,5 +1,6 @@
 // s512_partition.hip -- Partition512 (sort512.hpp:5729-5910, sort512kv.hpp:
-// 5958-6078) as a GPU kernel pair.
+// 5958-6078): one read and one write of every key, plus one copy of the
+// smaller side.
 //
 // Contract kept from the reference: the inclusive range [left, right] is
 // rearranged so that every key `<= pivot` (int32: signed compare; double: the
@@ -8,141 +9,325 @@
 // each side is unspecified in the reference; here it is stable (both sides keep
 // their input order), which is one of the arrangements the reference allows.
 //
-// Kernels:
-//   k_part_count    per chunk: number of keys <= pivot
-//   k_part_offsets  one block: exclusive scan of the chunk counts
-//   k_part_scatter  per chunk, in rounds of P*E keys: wave ballot -> lane rank,
-//                   per-(slot, wave) counts scanned in LDS, keys written to the
-//                   scratch array (low side from 0, high side from #low)
-// followed by one device-to-device copy of the range back into the caller's
-// array.  Keys are read twice and written twice.
+// The reference reads and writes every key once per pass (sort512.hpp:
+// 5748-5772, 5840-5864): it compress-stores each vector's low keys at the left
+// write front and its high keys at the right one.  On the GPU:
+//   k_part_sample    one block: the share of keys <= pivot in 4096 sampled
+//                    keys picks the direction (the larger side stays in place)
+//   k_part_pass      chunks in ticket order (from the left when the low side
+//                    is larger, else from the right): wave ballots rank every
+//                    key inside the chunk; a wave-wide decoupled look-back
+//                    over two status words per chunk (low and high counts of
+//                    the chunks before it) gives the chunk's offsets.  The
+//                    larger side is written in place -- a chunk's keys of
+//                    that side land at or before (from the right: at or
+//                    after) its own position, over keys that every earlier
+//                    chunk has already read -- and the smaller side to
+//                    scratch.
+//   k_part_copyback  the smaller side from scratch to its final place.
+// HBM traffic per key: 2 x key bytes, plus 2 x key bytes for the smaller side
+// (at most 24 B per fp64 key); count keys are never read twice.
 #include "s512_common.hpp"
 #include "s512_runtime.hpp"
 #include "s512_hostio.hpp"
 #include "../../include/s512.h"
+#include <algorithm>
 
 namespace s512 {
 
-constexpr int PART_P = 256;
-constexpr int PART_E = 16;
-constexpr int PART_ROUND = PART_P * PART_E;
-constexpr int PART_NW = PART_P / 64;
+constexpr int PT_P = 256;
+constexpr int PT_NW = PT_P / 64;
+constexpr uint32_t PT_SPIN_LIMIT = 1u << 20;
+template <class T>
+struct PartCfg {
+    static constexpr int E = sizeof(T) == 8 ? 16 : 32;
+    static constexpr uint32_t CH = PT_P * E;
+};
+
+// Device control block of one call (zeroed before every call).
+struct PartCtl {
+    uint32_t ticket;
+    uint32_t dir;        // 1: from the left (low side in place), 2: from the right
+    uint32_t timeout;    // a look-back wait gave up
+    uint32_t pad;
+    uint64_t total_low;  // written by the last chunk
+    uint64_t total_high;
+};
+
+__device__ __forceinline__ uint64_t pt_load(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void pt_store(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// status word: 0 = not published, else ((v + 1) << 1) | inclusive
+__device__ __forceinline__ uint64_t pt_pack(uint64_t v, bool incl) { return ((v + 1) << 1) | (incl ? 1u : 0u); }
+__device__ __forceinline__ uint64_t pt_val(uint64_t w) { return (w >> 1) - 1; }
+__device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        const uint32_t lo = __shfl_xor((uint32_t)v, d), hi = __shfl_xor((uint32_t)(v >> 32), d);
+        v += ((uint64_t)hi << 32) | lo;
+    }
+    return v;
+}
 
 template <class T>
 __device__ __forceinline__ bool le_pivot(T x, T p) { return x <= p; }
 
 template <class T>
-__global__ __launch_bounds__(PART_P) void k_part_count(const T* a, uint64_t n, uint64_t chunk, T pivot, uint64_t* cnt) {
-    __shared__ uint32_t wsum[PART_NW];
-    const uint64_t lo = blockIdx.x * chunk;
-    const uint64_t hi = lo + chunk < n ? lo + chunk : n;
+__global__ __launch_bounds__(256) void k_part_sample(const T* a, uint64_t n, T pivot, PartCtl* ctl) {
+    __shared__ uint32_t wsum[4];
     uint32_t c = 0;
-    uint64_t i = lo + threadIdx.x;
-    for (; i + 3 * PART_P < hi; i += 4 * PART_P) {
-        const T x0 = a[i], x1 = a[i + PART_P], x2 = a[i + 2 * PART_P], x3 = a[i + 3 * PART_P];
-        c += (le_pivot(x0, pivot) ? 1u : 0u) + (le_pivot(x1, pivot) ? 1u : 0u) + (le_pivot(x2, pivot) ? 1u : 0u) +
-             (le_pivot(x3, pivot) ? 1u : 0u);
-    }
-    for (; i < hi; i += PART_P) c += le_pivot(a[i], pivot) ? 1u : 0u;
+    for (uint32_t i = threadIdx.x; i < 4096; i += 256) c += le_pivot(a[(uint64_t)i * n / 4096], pivot) ? 1u : 0u;
     for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d);
     if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = c;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint64_t s = 0;
-        for (int w = 0; w < PART_NW; ++w) s += wsum[w];
-        cnt[blockIdx.x] = s;
-    }
+    if (threadIdx.x == 0) ctl->dir = (wsum[0] + wsum[1] + wsum[2] + wsum[3]) * 2 >= 4096 ? 1u : 2u;
 }
 
-// cnt[c] -> exclusive prefix of low counts (in place); total low in cnt[nc].
-__global__ __launch_bounds__(1024) void k_part_offsets(uint64_t* cnt, uint32_t nc) {
-    __shared__ uint64_t part[1024];
-    // sequential per thread over a strip, then a scan of the strip sums
-    const uint32_t per = (nc + 1023) / 1024;
-    const uint32_t b = threadIdx.x * per, e = b + per < nc ? b + per : nc;
-    uint64_t s = 0;
-    for (uint32_t i = b; i < e; ++i) s += cnt[i];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint64_t run = 0;
-        for (int i = 0; i < 1024; ++i) {
-            const uint64_t v = part[i];
-            part[i] = run;
-            run += v;
-        }
-        cnt[nc] = run;
-    }
-    __syncthreads();
-    uint64_t run = part[threadIdx.x];
-    for (uint32_t i = b; i < e; ++i) {
-        const uint64_t v = cnt[i];
-        cnt[i] = run;
-        run += v;
-    }
-}
-
+// Chunk c (ticket order) covers, from the left, [c CH, (c+1) CH) of the
+// range; from the right, [n - (c+1) CH, n - c CH).  `out` holds out_cap keys:
+// the smaller side, from its start (left pass: high keys) or its end (right
+// pass: low keys).  status: two words per chunk, zero before the launch.
 template <class T, bool KV>
-__global__ __launch_bounds__(PART_P) void k_part_scatter(const T* a, const int32_t* va, uint64_t n, uint64_t chunk,
-                                                         T pivot, const uint64_t* off, T* out, int32_t* vout) {
-    __shared__ uint32_t tab[PART_E][PART_NW];  // per (slot, wave) low counts, then their prefix
-    __shared__ uint32_t round_low;
+__global__ __launch_bounds__(PT_P) void k_part_pass(T* a, int32_t* va, uint64_t n, T pivot, T* out, int32_t* vout,
+                                                    uint64_t out_cap, uint64_t* status, PartCtl* ctl,
+                                                    uint32_t nchunks) {
+    constexpr int E = PartCfg<T>::E;
+    constexpr uint32_t CH = PartCfg<T>::CH;
+    __shared__ uint32_t tab[E][PT_NW];  // low keys per (slot, wave), then their exclusive prefix
+    __shared__ uint32_t s_work, s_low;
+    __shared__ uint64_t s_pre[2];
+    __shared__ uint32_t s_bad;
     const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
-    const uint64_t lo = blockIdx.x * chunk;
-    const uint64_t hi = lo + chunk < n ? lo + chunk : n;
-    const uint64_t total_low = off[gridDim.x];
-    uint64_t low_run = off[blockIdx.x];   // low keys before this point
-    uint64_t high_run = lo - low_run;     // high keys before this point
-    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (uint64_t base = lo; base < hi; base += PART_ROUND) {
-        T x[PART_E];
-        int32_t xv[PART_E];
-        uint64_t bal[PART_E];
-        bool f[PART_E];
+    const uint64_t lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
+    const bool fwd = ctl->dir == 1u;
+    for (;;) {
+        __syncthreads();  // the previous chunk is done with LDS
+        if (t == 0) s_work = atomicAdd(&ctl->ticket, 1u);
+        __syncthreads();
+        const uint32_t c = s_work;
+        if (c >= nchunks) break;  // uniform
+        uint64_t lo, hi;
+        if (fwd) {
+            lo = (uint64_t)c * CH;
+            hi = lo + CH < n ? lo + CH : n;
+        } else {
+            hi = n - (uint64_t)c * CH;
+            lo = hi > CH ? hi - CH : 0;
+        }
+        const uint32_t cnt = (uint32_t)(hi - lo);
+        T x[E];
+        int32_t xv[KV ? E : 1];
+        bool f[E];
+        uint64_t bal[E];
 #pragma unroll
-        for (int e = 0; e < PART_E; ++e) {
-            const uint64_t i = base + (uint64_t)e * PART_P + t;
-            const bool in = i < hi;
-            const uint64_t ic = in ? i : hi - 1;  // in bounds, unconditional load
-            x[e] = a[ic];
-            if constexpr (KV) xv[e] = va[ic];
-            f[e] = in && le_pivot(x[e], pivot);
+        for (int e = 0; e < E; ++e) {
+            const uint32_t i = e * PT_P + t, ic = i < cnt ? i : cnt - 1;  // in bounds, unconditional
+            x[e] = a[lo + ic];
+            if constexpr (KV) xv[e] = va[lo + ic];
         }
 #pragma unroll
-        for (int e = 0; e < PART_E; ++e) {
+        for (int e = 0; e < E; ++e) {
+            f[e] = (e * PT_P + t < cnt) && le_pivot(x[e], pivot);
             bal[e] = __ballot(f[e]);
             if (lane == 0) tab[e][w] = (uint32_t)__popcll(bal[e]);
         }
-        __syncthreads();
+        if (t == 0) s_bad = 0;
+        __syncthreads();  // every key of the chunk is in registers
         if (t == 0) {
             uint32_t run = 0;
-            for (int e = 0; e < PART_E; ++e)
-                for (int ww = 0; ww < PART_NW; ++ww) {
+            for (int e = 0; e < E; ++e)
+                for (int ww = 0; ww < PT_NW; ++ww) {
                     const uint32_t v = tab[e][ww];
                     tab[e][ww] = run;
                     run += v;
                 }
-            round_low = run;
+            s_low = run;
         }
         __syncthreads();
+        const uint32_t low_c = s_low, high_c = cnt - low_c;
+        if (w == 0) {
+            uint64_t* me = status + 2 * (uint64_t)c;
+            uint64_t pl = 0, ph = 0;
+            bool tmo = false;
+            if (lane == 0) {
+                pt_store(me, pt_pack(low_c, c == 0));
+                pt_store(me + 1, pt_pack(high_c, c == 0));
+            }
+            if (c > 0) {
+                // lane i reads chunk q - i; the walk ends at the nearest chunk
+                // with an inclusive prefix (chunk 0 always publishes one)
+                bool dl = false, dh = false;
+                for (int64_t q = (int64_t)c - 1; !(dl && dh) && !tmo; q -= 64) {
+                    const int64_t r = q - (int64_t)lane;
+                    const bool ok = r >= 0;
+                    uint64_t wl = 0, wh = 0;
+                    if (ok) {
+                        wl = pt_load(status + 2 * r);
+                        wh = pt_load(status + 2 * r + 1);
+                    }
+                    uint64_t ml = 0, mh = 0;
+                    bool fl = false, fh = false;
+                    for (uint32_t it = 1;; ++it) {
+                        // lanes up to and including the first inclusive one
+                        const uint64_t il = __ballot(!ok || (wl & 1)), ih = __ballot(!ok || (wh & 1));
+                        const uint64_t bl = il & (~il + 1), bh = ih & (~ih + 1);
+                        fl = il != 0;
+                        fh = ih != 0;
+                        ml = fl ? (bl | (bl - 1)) : ~0ull;
+                        mh = fh ? (bh | (bh - 1)) : ~0ull;
+                        const uint64_t pend = (dl ? 0ull : (__ballot(ok && wl == 0) & ml)) |
+                                              (dh ? 0ull : (__ballot(ok && wh == 0) & mh));
+                        if (!pend) break;
+                        if ((it & 255u) == 0) {
+                            if (it >= PT_SPIN_LIMIT && lane == 0) atomicOr(&ctl->timeout, 1u);
+                            if (__hip_atomic_load(&ctl->timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                                tmo = true;
+                                break;
+                            }
+                        }
+                        __builtin_amdgcn_s_sleep(2);
+                        if (ok && wl == 0) wl = pt_load(status + 2 * r);
+                        if (ok && wh == 0) wh = pt_load(status + 2 * r + 1);
+                    }
+                    if (tmo) break;
+                    if (!dl) {
+                        pl += wave_sum64(ok && ((ml >> lane) & 1) ? pt_val(wl) : 0ull);
+                        dl = fl;
+                    }
+                    if (!dh) {
+                        ph += wave_sum64(ok && ((mh >> lane) & 1) ? pt_val(wh) : 0ull);
+                        dh = fh;
+                    }
+                }
+                if (lane == 0 && !tmo) {
+                    pt_store(me, pt_pack(pl + low_c, true));
+                    pt_store(me + 1, pt_pack(ph + high_c, true));
+                }
+            }
+            if (lane == 0) {
+                s_pre[0] = pl;
+                s_pre[1] = ph;
+                if (tmo) s_bad = 1;
+                if (c + 1 == nchunks) {
+                    ctl->total_low = pl + low_c;
+                    ctl->total_high = ph + high_c;
+                }
+            }
+        }
+        __syncthreads();
+        if (s_bad) continue;  // nothing is written by a chunk without its offsets
+        const uint64_t PL = s_pre[0], PH = s_pre[1];
 #pragma unroll
-        for (int e = 0; e < PART_E; ++e) {
-            const uint64_t i = base + (uint64_t)e * PART_P + t;
-            if (i >= hi) continue;
-            const uint32_t lr = tab[e][w] + (uint32_t)__popcll(bal[e] & lt_mask);  // low keys before i in round
-            const uint64_t in_round = (uint64_t)e * PART_P + t;                     // keys before i in round
+        for (int e = 0; e < E; ++e) {
+            const uint32_t i = e * PT_P + t;
+            if (i >= cnt) continue;
+            const uint32_t lr = tab[e][w] + (uint32_t)__popcll(bal[e] & lt_mask);  // low keys before slot i
+            const uint32_t hr = i - lr;                                             // high keys before slot i
             uint64_t d;
-            if (f[e]) d = low_run + lr;
-            else d = total_low + high_run + (in_round - lr);
-            out[d] = x[e];
-            if constexpr (KV) vout[d] = xv[e];
+            bool in_place;
+            if (fwd) {
+                in_place = f[e];
+                d = f[e] ? PL + lr : PH + hr;
+            } else {
+                in_place = !f[e];
+                d = f[e] ? out_cap - PL - low_c + lr : n - PH - high_c + hr;
+            }
+            if (in_place) {
+                if (d < n) {
+                    a[d] = x[e];
+                    if constexpr (KV) va[d] = xv[e];
+                }
+            } else if (d < out_cap) {
+                out[d] = x[e];
+                if constexpr (KV) vout[d] = xv[e];
+            }
         }
-        const uint32_t rl = round_low;
-        const uint64_t cnt = (hi - base) < (uint64_t)PART_ROUND ? (hi - base) : (uint64_t)PART_ROUND;
-        low_run += rl;
-        high_run += cnt - rl;
-        __syncthreads();
     }
+}
+
+// The smaller side from scratch to its place: after the low side (left pass)
+// or at the start of the range (right pass).
+template <class T, bool KV>
+__global__ __launch_bounds__(256) void k_part_copyback(T* a, int32_t* va, uint64_t n, const T* out,
+                                                       const int32_t* vout, uint64_t out_cap, const PartCtl* ctl) {
+    if (ctl->timeout) return;
+    const bool fwd = ctl->dir == 1u;
+    const uint64_t m = fwd ? ctl->total_high : ctl->total_low;
+    const uint64_t src = fwd ? 0 : out_cap - m, dst = fwd ? ctl->total_low : 0;
+    if (m > out_cap || dst + m > n) return;
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < m; i += (uint64_t)gridDim.x * 256ull) {
+        a[dst + i] = out[src + i];
+        if constexpr (KV) va[dst + i] = vout[src + i];
+    }
+}
+
+__global__ void k_part_zero(uint64_t* p, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+// Bytes of workspace a partition of n keys needs: control block, status
+// words, then the scratch side (n keys, and n values for key/value).
+template <class T, bool KV>
+static size_t part_workspace(uint64_t n, uint32_t* nchunks, size_t* st_off, size_t* key_off, size_t* val_off) {
+    const uint32_t nc = (uint32_t)((n + PartCfg<T>::CH - 1) / PartCfg<T>::CH);
+    size_t off = 256;  // PartCtl
+    *st_off = off;
+    off += ((size_t)nc * 2 * sizeof(uint64_t) + 255) & ~(size_t)255;
+    *key_off = off;
+    off += (n * sizeof(T) + 255) & ~(size_t)255;
+    *val_off = off;
+    if (KV) off += (n * sizeof(int32_t) + 255) & ~(size_t)255;
+    *nchunks = nc;
+    return off;
+}
+
+// Partition of a[left..right] (device memory) with the workspace `ws` (at
+// least part_workspace() bytes); *out_bound = left + #{<= pivot}.  Reads the
+// bound back (synchronises the stream); calls no allocator.
+template <class T, bool KV>
+static s512_status partition_run(T* a, int32_t* va, size_t left, size_t right, T pivot, size_t* out_bound,
+                                 hipStream_t st, char* ws, uint64_t* host_word) {
+    const uint64_t n = (uint64_t)right - left + 1;
+    uint32_t nc;
+    size_t st_off, key_off, val_off;
+    part_workspace<T, KV>(n, &nc, &st_off, &key_off, &val_off);
+    PartCtl* ctl = (PartCtl*)ws;
+    uint64_t* status = (uint64_t*)(ws + st_off);
+    T* out = (T*)(ws + key_off);
+    int32_t* vout = KV ? (int32_t*)(ws + val_off) : nullptr;
+    T* src = a + left;
+    int32_t* vsrc = KV ? va + left : nullptr;
+    int dev = 0, cus = 0, occ = 0;
+    S512_HIP(hipGetDevice(&dev));
+    S512_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    S512_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_part_pass<T, KV>, PT_P, 0));
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(nc, (uint64_t)std::max(occ, 1) * cus);
+    // control block and status words start at zero (a fill kernel: callers may capture this)
+    const uint64_t zw = (st_off + (size_t)nc * 2 * sizeof(uint64_t)) / sizeof(uint64_t);
+    k_part_zero<<<(unsigned)std::min<uint64_t>((zw + 255) / 256, 1024), 256, 0, st>>>((uint64_t*)ws, zw);
+    S512_CHECK_LAUNCH();
+    k_part_sample<T><<<1, 256, 0, st>>>(src, n, pivot, ctl);
+    S512_CHECK_LAUNCH();
+    k_part_pass<T, KV><<<grid, PT_P, 0, st>>>(src, vsrc, n, pivot, out, vout, n, status, ctl, nc);
+    S512_CHECK_LAUNCH();
+    k_part_copyback<T, KV><<<(unsigned)std::min<uint64_t>((n + 255) / 256, 4096), 256, 0, st>>>(src, vsrc, n, out,
+                                                                                               vout, n, ctl);
+    S512_CHECK_LAUNCH();
+    uint64_t res[2];
+    S512_HIP(hipMemcpyAsync(host_word, &ctl->timeout, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    S512_HIP(hipMemcpyAsync(host_word + 1, &ctl->total_low, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    S512_HIP(hipStreamSynchronize(st));
+    memcpy(res, host_word, sizeof(res));
+    if ((uint32_t)res[0]) {
+        set_error("Partition512: a look-back wait did not end (the range is left partly rearranged)");
+        return S512_ERR_INTERNAL;
+    }
+    *out_bound = left + (size_t)res[1];
+    return S512_OK;
 }
 
 template <class T, bool KV>
@@ -157,37 +342,13 @@ static s512_status partition_dev(T* a, int32_t* va, size_t left, size_t right, T
     if (rc) return rc;
     DeviceState& ds = device_state(dev);
     std::lock_guard<std::mutex> lk(ds.mu);
-    uint64_t chunk = (n + 2047) / 2048;
-    chunk = (chunk + PART_ROUND - 1) / PART_ROUND * PART_ROUND;
-    const uint32_t nc = (uint32_t)((n + chunk - 1) / chunk);
-    const size_t dbytes = n * sizeof(T) + (KV ? n * sizeof(int32_t) : 0);
-    if ((rc = ds.data.reserve(dbytes, st))) return rc;
-    if ((rc = ds.arena.reserve((nc + 1) * sizeof(uint64_t), st))) return rc;
+    uint32_t nc;
+    size_t st_off, key_off, val_off;
+    const size_t bytes = part_workspace<T, KV>(n, &nc, &st_off, &key_off, &val_off);
+    if ((rc = ds.data.reserve(bytes, st))) return rc;
     if ((rc = ds.pinned2.reserve(4096, st))) return rc;
-    T* out = (T*)ds.data.ptr;
-    int32_t* vout = KV ? (int32_t*)((char*)ds.data.ptr + n * sizeof(T)) : nullptr;
-    uint64_t* cnt = (uint64_t*)ds.arena.ptr;
-    T* src = a + left;
-    int32_t* vsrc = KV ? va + left : nullptr;
-    k_part_count<T><<<nc, PART_P, 0, st>>>(src, n, chunk, pivot, cnt);
-    S512_CHECK_LAUNCH();
-    k_part_offsets<<<1, 1024, 0, st>>>(cnt, nc);
-    S512_CHECK_LAUNCH();
-    k_part_scatter<T, KV><<<nc, PART_P, 0, st>>>(src, vsrc, n, chunk, pivot, cnt, out, vout);
-    S512_CHECK_LAUNCH();
-    S512_HIP(hipMemcpyAsync(src, out, n * sizeof(T), hipMemcpyDeviceToDevice, st));
-    if (KV) S512_HIP(hipMemcpyAsync(vsrc, vout, n * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
-    uint64_t* h = (uint64_t*)ds.pinned2.ptr;
-    S512_HIP(hipMemcpyAsync(h, cnt + nc, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-    S512_HIP(hipStreamSynchronize(st));
-    *out_bound = left + (size_t)h[0];
-    return S512_OK;
-}
-
-// host arrays: stage [0, right] (the reference indexes from the base pointer)
-template <class T, class Fn>
-static s512_status on_device_range(T* a, size_t right, Fn fn) {
-    return with_staged_array(a, right + 1, fn);
+    return partition_run<T, KV>(a, va, left, right, pivot, out_bound, st, (char*)ds.data.ptr,
+                                (uint64_t*)ds.pinned2.ptr);
 }
 
 }  // namespace s512
@@ -206,20 +367,29 @@ extern "C" s512_status s512kv_partition_i32_dev(int32_t* d_k, int32_t* d_v, size
                                                 size_t* out_bound, void* stream) {
     return partition_dev<int32_t, true>(d_k, d_v, left, right, pivot, out_bound, (hipStream_t)stream);
 }
+// Host arrays: only [left, right] is staged; the bound is shifted back.
 extern "C" s512_status s512_partition_i32(int32_t* a, size_t left, size_t right, int32_t pivot, size_t* out_bound) {
     if (!out_bound) { set_error("null out_bound"); return S512_ERR_INVALID; }
     if (right + 1 <= left) { *out_bound = left; return S512_OK; }
     if (!a) { set_error("null array"); return S512_ERR_INVALID; }
-    return on_device_range(a, right, [&](int32_t* d, hipStream_t st) {
-        return s512_partition_i32_dev(d, left, right, pivot, out_bound, st);
+    const size_t n = right - left + 1;
+    return with_staged_array(a + left, n, [&](int32_t* d, hipStream_t st) {
+        size_t b = 0;
+        s512_status rc = s512_partition_i32_dev(d, 0, n - 1, pivot, &b, st);
+        *out_bound = left + b;
+        return rc;
     });
 }
 extern "C" s512_status s512_partition_f64(double* a, size_t left, size_t right, double pivot, size_t* out_bound) {
     if (!out_bound) { set_error("null out_bound"); return S512_ERR_INVALID; }
     if (right + 1 <= left) { *out_bound = left; return S512_OK; }
     if (!a) { set_error("null array"); return S512_ERR_INVALID; }
-    return on_device_range(a, right, [&](double* d, hipStream_t st) {
-        return s512_partition_f64_dev(d, left, right, pivot, out_bound, st);
+    const size_t n = right - left + 1;
+    return with_staged_array(a + left, n, [&](double* d, hipStream_t st) {
+        size_t b = 0;
+        s512_status rc = s512_partition_f64_dev(d, 0, n - 1, pivot, &b, st);
+        *out_bound = left + b;
+        return rc;
     });
 }
 extern "C" s512_status s512kv_partition_i32(int32_t* k, int32_t* v, size_t left, size_t right, int32_t pivot,
@@ -227,7 +397,11 @@ extern "C" s512_status s512kv_partition_i32(int32_t* k, int32_t* v, size_t left,
     if (!out_bound) { set_error("null out_bound"); return S512_ERR_INVALID; }
     if (right + 1 <= left) { *out_bound = left; return S512_OK; }
     if (!k || !v) { set_error("null array"); return S512_ERR_INVALID; }
-    return with_staged_pair(k, v, right + 1, [&](int32_t* dk, int32_t* dv, hipStream_t st) {
-        return s512kv_partition_i32_dev(dk, dv, left, right, pivot, out_bound, st);
+    const size_t n = right - left + 1;
+    return with_staged_pair(k + left, v + left, n, [&](int32_t* dk, int32_t* dv, hipStream_t st) {
+        size_t b = 0;
+        s512_status rc = s512kv_partition_i32_dev(dk, dv, 0, n - 1, pivot, &b, st);
+        *out_bound = left + b;
+        return rc;
     });
 }

@@ -53,6 +53,71 @@ def test_partition_large_subrange(s512, orc, dev, n):
         assert np.array_equal(a[b:right + 1], seg0[seg0 > pivot])
 
 
+@pytest.mark.parametrize("low_share", [0.02, 0.3, 0.5, 0.7, 0.98])
+@pytest.mark.parametrize("dt", [np.int32, np.float64])
+def test_partition_both_directions(s512, orc, dev, low_share, dt):
+    """The larger side stays in place (a pass from the left when most keys are
+    <= pivot, from the right otherwise), the smaller one goes through
+    scratch: bound, both sides, stability and the untouched outside."""
+    n = 2_000_003
+    rng = np.random.default_rng(int(low_share * 1000))
+    a0 = rng.integers(0, 10**6, n + 200).astype(dt)
+    pivot = dt(int(low_share * 10**6))
+    left, right = 100, n + 99
+    t = to_dev(a0, dev)
+    b = s512.partition(t, left, right, pivot)
+    a = to_host(t)
+    _check_partition(a0, a, left, right, pivot, b)
+    seg0 = a0[left:right + 1]
+    assert np.array_equal(a[left:b], seg0[seg0 <= pivot])
+    assert np.array_equal(a[b:right + 1], seg0[seg0 > pivot])
+
+
+@pytest.mark.parametrize("low_at_samples", [True, False])
+def test_partition_misleading_sample(s512, orc, dev, low_at_samples):
+    """The direction comes from 4096 evenly spaced keys; an input whose keys at
+    exactly those places belong to the smaller side sends the larger side
+    through scratch.  The result must not depend on it."""
+    n = 1 << 22
+    a0 = np.full(n, 5.0 if low_at_samples else -5.0)
+    idx = (np.arange(4096, dtype=np.uint64) * np.uint64(n) // np.uint64(4096)).astype(np.int64)
+    a0[idx] = -5.0 if low_at_samples else 5.0
+    t = to_dev(a0, dev)
+    b = s512.partition(t, 0, n - 1, 0.0)
+    a = to_host(t)
+    _check_partition(a0, a, 0, n - 1, 0.0, b)
+
+
+def test_partition_kv_multichunk(s512, orc, dev):
+    rng = np.random.default_rng(8)
+    n = 3_000_001
+    k0 = rng.integers(-1000, 1000, n).astype(np.int32)
+    v0 = np.arange(n, dtype=np.int32)
+    for pivot in (-900, 0, 900):
+        tk, tv = to_dev(k0, dev), to_dev(v0, dev)
+        b = s512.kv_partition(tk, tv, 0, n - 1, pivot)
+        k, v = to_host(tk), to_host(tv)
+        _check_partition(k0, k, 0, n - 1, pivot, b)
+        lo = k0 <= pivot
+        assert np.array_equal(v[:b], v0[lo]) and np.array_equal(v[b:], v0[~lo])  # stable pairs
+
+
+def test_partition_full_size_f64(s512, orc, dev):
+    """268,435,456 fp64 keys, one pivot (the median of a uniform input): the
+    bound by count, each side on its own side, the multiset by hash."""
+    import torch
+    n = 1 << 28
+    t = torch.empty(n, dtype=torch.float64, device=dev)
+    s512.fill(t, "uniform", seed=3)
+    pivot = 0.5
+    want = int((t <= pivot).sum().item())
+    h0 = int((t.view(torch.int64) * -7046029254386353131).sum().item())
+    b = s512.partition(t, 0, n - 1, pivot)
+    assert b == want
+    assert bool((t[:b] <= pivot).all().item()) and not bool((t[b:] <= pivot).any().item())
+    assert int((t.view(torch.int64) * -7046029254386353131).sum().item()) == h0
+
+
 def test_partition_edge_cases(s512, orc, dev):
     a0 = np.arange(10, dtype=np.int32)
     t = to_dev(a0, dev)
