@@ -32,6 +32,8 @@
 #include "s512_hostio.hpp"
 #include "../../include/s512.h"
 #include <algorithm>
+#include <map>
+#include <memory>
 
 namespace s512 {
 
@@ -351,6 +353,74 @@ static s512_status partition_dev(T* a, int32_t* va, size_t left, size_t right, T
                                 (uint64_t*)ds.pinned2.ptr);
 }
 
+// ---------------------------------------------------------------- no allocation
+// A fixed per-device workspace for callers that cannot allocate (code under
+// hipGraph capture, serving loops, harnesses that forbid hipMalloc).  The
+// array lives in every GPU the library runs on until the process ends; its
+// device address is looked up once per device with hipGetSymbolAddress (the
+// name itself is a host-side stand-in, not a device address).  Calls on one
+// GPU share it: calls that overlap on one GPU race.
+constexpr size_t PART_SCRATCH_BYTES = 64u << 20;
+static __device__ __align__(256) char scratch[PART_SCRATCH_BYTES];
+
+struct ScratchSlot {
+    std::mutex mu;
+    char* ptr = nullptr;
+    uint64_t* host = nullptr;  // the call's result words (ordinary host memory, owned here)
+    uint64_t words[4];
+};
+static ScratchSlot& scratch_slot(int dev) {
+    static std::mutex map_mu;
+    static std::map<int, std::unique_ptr<ScratchSlot>> slots;
+    std::lock_guard<std::mutex> lk(map_mu);
+    auto& p = slots[dev];
+    if (!p) p.reset(new ScratchSlot());
+    return *p;
+}
+
+// The largest range whose workspace fits the fixed array (8,384,480 fp64 keys).
+template <class T, bool KV>
+static uint64_t part_max_keys() {
+    uint64_t lo = 0, hi = PART_SCRATCH_BYTES;  // part_workspace(lo) fits, part_workspace(hi) does not
+    while (hi - lo > 1) {
+        const uint64_t mid = lo + (hi - lo) / 2;
+        uint32_t nc;
+        size_t s, k, v;
+        (part_workspace<T, KV>(mid, &nc, &s, &k, &v) <= PART_SCRATCH_BYTES ? lo : hi) = mid;
+    }
+    return lo;
+}
+
+template <class T, bool KV>
+static s512_status partition_noalloc(T* a, int32_t* va, size_t left, size_t right, T pivot, size_t* out_bound,
+                                     hipStream_t st) {
+    if (!out_bound) { set_error("null out_bound"); return S512_ERR_INVALID; }
+    if (right + 1 <= left) { *out_bound = left; return S512_OK; }
+    if (!a || (KV && !va)) { set_error("null array"); return S512_ERR_INVALID; }
+    const uint64_t n = (uint64_t)right - left + 1;
+    uint32_t nc;
+    size_t st_off, key_off, val_off;
+    const size_t bytes = part_workspace<T, KV>(n, &nc, &st_off, &key_off, &val_off);
+    if (bytes > PART_SCRATCH_BYTES) {
+        set_error("Partition512 without allocation: %llu keys need %zu bytes of workspace, the fixed one holds %zu "
+                  "(at most %llu keys of this type)",
+                  (unsigned long long)n, bytes, PART_SCRATCH_BYTES, (unsigned long long)part_max_keys<T, KV>());
+        return S512_ERR_TOO_LARGE;
+    }
+    int dev;
+    s512_status rc = current_device(&dev);
+    if (rc) return rc;
+    ScratchSlot& sl = scratch_slot(dev);
+    std::lock_guard<std::mutex> lk(sl.mu);
+    if (!sl.ptr) {
+        void* p = nullptr;
+        S512_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        sl.ptr = (char*)p;
+        sl.host = sl.words;
+    }
+    return partition_run<T, KV>(a, va, left, right, pivot, out_bound, st, sl.ptr, sl.host);
+}
+
 }  // namespace s512
 
 using namespace s512;
@@ -367,6 +437,11 @@ extern "C" s512_status s512kv_partition_i32_dev(int32_t* d_k, int32_t* d_v, size
                                                 size_t* out_bound, void* stream) {
     return partition_dev<int32_t, true>(d_k, d_v, left, right, pivot, out_bound, (hipStream_t)stream);
 }
+extern "C" s512_status s512_partition_f64_dev_noalloc(double* d_a, size_t left, size_t right, double pivot,
+                                                      size_t* out_bound, void* stream) {
+    return partition_noalloc<double, false>(d_a, nullptr, left, right, pivot, out_bound, (hipStream_t)stream);
+}
+
 // Host arrays: only [left, right] is staged; the bound is shifted back.
 extern "C" s512_status s512_partition_i32(int32_t* a, size_t left, size_t right, int32_t pivot, size_t* out_bound) {
     if (!out_bound) { set_error("null out_bound"); return S512_ERR_INVALID; }

@@ -94,6 +94,8 @@ _SIGS = {
     "s512_partition_f64": ([_vp, _c_size, _c_size, ctypes.c_double, ctypes.POINTER(_c_size)], ctypes.c_int),
     "s512_partition_i32_dev": ([_vp, _c_size, _c_size, ctypes.c_int32, ctypes.POINTER(_c_size), _vp], ctypes.c_int),
     "s512_partition_f64_dev": ([_vp, _c_size, _c_size, ctypes.c_double, ctypes.POINTER(_c_size), _vp], ctypes.c_int),
+    "s512_partition_f64_dev_noalloc": ([_vp, _c_size, _c_size, ctypes.c_double, ctypes.POINTER(_c_size), _vp],
+                                       ctypes.c_int),
     "s512kv_partition_i32": ([_vp, _vp, _c_size, _c_size, ctypes.c_int32, ctypes.POINTER(_c_size)], ctypes.c_int),
     "s512kv_partition_i32_dev": ([_vp, _vp, _c_size, _c_size, ctypes.c_int32, ctypes.POINTER(_c_size), _vp],
                                  ctypes.c_int),
@@ -397,6 +399,14 @@ def partition(a, left, right, pivot):
                                                        _stream_of(a)))
     else:
         _check(getattr(lib, f"s512_partition_{k}")(_ptr(a), left, right, cast(pivot), ctypes.byref(out)))
+    return out.value
+
+
+def partition_noalloc(a, left, right, pivot):
+    """Partition512 of a float64 device tensor with the library's fixed
+    workspace (no allocation; at most about 8.3 million keys per call)."""
+    out = _c_size()
+    _check(lib.s512_partition_f64_dev_noalloc(_ptr(a), left, right, float(pivot), ctypes.byref(out), _stream_of(a)))
     return out.value
 
 

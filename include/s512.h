@@ -145,6 +145,16 @@ s512_status s512_partition_i32_dev(int32_t* d_a, size_t left, size_t right, int3
                                    void* stream);
 s512_status s512_partition_f64_dev(double* d_a, size_t left, size_t right, double pivot, size_t* out_bound,
                                    void* stream);
+/* No reference counterpart: s512_partition_f64_dev for callers that cannot
+ * allocate (code under hipGraph capture, serving loops, harnesses that forbid
+ * hipMalloc).  Same result; its workspace is a fixed 64 MiB array that every
+ * GPU the library runs on holds until the process ends, so it partitions at
+ * most 8,384,480 keys per call (larger ranges return S512_ERR_TOO_LARGE with
+ * nothing changed) and calls no allocator.  Calls
+ * on one GPU share that array: calls that overlap on one GPU race.  Device
+ * pointers; runs on `stream` and reads the bound back.                     */
+s512_status s512_partition_f64_dev_noalloc(double* d_a, size_t left, size_t right, double pivot,
+                                           size_t* out_bound, void* stream);
 
 /* ------------------------------------------------------------ small sort */
 /* Sort512::SmallSort16V(double*, len <= 128)  sort512.hpp:2255
