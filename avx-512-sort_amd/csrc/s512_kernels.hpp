@@ -1190,6 +1190,245 @@ __global__ __launch_bounds__(P) void k_scatter(const PartSeg* ps, const ChunkRef
     }
 }
 
+// ---------------------------------------------------------------- line-buffered scatter
+// Keys only.  The rounds of k_scatter (classify or stored ids, LDS ranks,
+// staging grouped by bucket), but a 128-byte line of an ordinary bucket's run
+// is stored only once it is complete: the keys of the bucket's unfinished last
+// line wait in an LDS line buffer (one per ordinary bucket, indexed by the
+// key's place in its line), and in the round whose keys complete the line they
+// are stored just before those keys, so the line reaches the L2 as back-to-back
+// stores.  Measured on MI355X (2 GiB, up to 4096 sequential streams per CU):
+// whole 128-byte lines, or two halves stored back to back, stream at
+// 4.3-4.9 TB/s; runs of 16 fp64 keys off line alignment, the pattern of a
+// plain multiway scatter, at 1.6-1.9 TB/s.  Only the first line of each
+// (chunk, bucket) run (shared with the previous chunk) and its last line at
+// the chunk's end are stored partially, so the planner gives this kernel
+// large chunks.  Equality buckets (odd) are stored directly.
+constexpr int SCATTER_LB = 4;                // scatter_shape of the line-buffered kernel
+constexpr uint64_t LB_MIN_CHUNK = 1u << 18;  // keys per chunk at least (partial lines ~2 per 1000)
+
+template <class T, bool IDS, int P, int E>
+struct ScatterLbLds {
+    using K = typename KeyTraits<T>::K;
+    static constexpr int ROUND = P * E;
+    static constexpr int LINE = 128 / (int)sizeof(T);  // keys per 128-byte line
+    static constexpr int LB = LINE - 1;                // waiting keys per ordinary bucket, at most
+    static constexpr bool CLS = !IDS;
+    K lsrt[CLS ? FMAX : 1];
+    __align__(16) uint32_t lguide[CLS ? LDS_TABLE_WORDS : 1];
+    uint32_t cursor[BMAX];  // next destination of each bucket (segment-relative)
+    uint32_t rh[BMAX];      // round histogram
+    uint32_t dbase[BMAX];   // destination of staged slot i is dbase[bucket] + i
+    uint16_t rs[BMAX];      // round start of each bucket in the staging area
+    uint32_t cut[FMAX];     // ordinary bucket 2o: keys placed at or past cut[o] wait this round
+    uint32_t head[FMAX];    // first line boundary of its run in this chunk
+    uint16_t nw[FMAX];      // keys waiting: the places [cursor - nw, cursor)
+    uint16_t nfl[FMAX];     // waiting keys whose line this round completes (stored before barrier 4)
+    uint32_t wtot[P / 64];
+    uint32_t dummy[DUMMY_SLOTS];
+    T stg[ROUND];
+    uint16_t stb[ROUND];
+    T lbuf[FMAX * LB];
+};
+
+template <class T, bool IDS, int P, int E>
+__global__ __launch_bounds__(P) void k_scatter_lb(const PartSeg* ps, const ChunkRef* cr, T* a, T* b,
+                                                  const typename KeyTraits<T>::K* trees, const uint32_t* guides,
+                                                  const uint32_t* hist, const uint64_t* bstart,
+                                                  const uint32_t* nchunk, const uint16_t* ids) {
+    using Tr = KeyTraits<T>;
+    using K = typename Tr::K;
+    using L = ScatterLbLds<T, IDS, P, E>;
+    constexpr int ROUND = L::ROUND, LINE = L::LINE, LB = L::LB;
+    constexpr int NW = P / 64;
+    constexpr int BPT = BMAX / P;  // buckets per thread in the scan (an even number: ordinary + equality)
+    static_assert(BPT * P == BMAX && BPT % 2 == 0 && ROUND <= 32768 && LINE >= 2, "line-buffered scatter shape");
+    __shared__ L sm;
+
+    for (uint32_t wi = blockIdx.x; wi < *nchunk; wi += gridDim.x) {
+    uint32_t wc = wi;
+    if (gridDim.x >= *nchunk) {  // XCD-contiguous runs of chunks, as in k_scatter
+        const uint32_t N = *nchunk, q = N / 8, r = N % 8, x = wi % 8;
+        wc = x * q + (x < r ? x : r) + wi / 8;
+    }
+    const ChunkRef c = cr[wc];
+    const PartSeg s = ps[c.seg];
+    const uint32_t B = 2u << s.logf;
+    const uint32_t t = threadIdx.x, lane = t & 63u, wid = t >> 6;
+    const T* src = (s.buf ? b : a) + s.begin;
+    T* dst = (s.buf ? a : b) + s.begin;
+    // line arithmetic on the element's address: dst need not be line aligned
+    const uint64_t dph = (uint64_t)(uintptr_t)dst / sizeof(T);
+    const uint16_t* isrc = IDS ? ids + s.begin : nullptr;
+    Guide g{0, 0, 0, 0, 0, 0};
+    if constexpr (!IDS) g = load_classifier<K, P>(s, trees, guides, sm.lsrt, sm.lguide);
+    const uint32_t* h = hist + s.hist_base + (uint64_t)c.idx * B;
+    for (uint32_t j = t; j < BMAX; j += P) {
+        const uint32_t cur = j < B ? (uint32_t)bstart[s.bkt_base + j] + h[j] : 0u;
+        sm.cursor[j] = cur;
+        sm.rh[j] = 0;
+        if (!(j & 1u)) {
+            sm.head[j >> 1] = (uint32_t)(((dph + cur + LINE - 1) & ~(uint64_t)(LINE - 1)) - dph);
+            sm.nw[j >> 1] = 0;
+        }
+    }
+    const uint64_t lo = (uint64_t)c.idx * s.chunk;
+    const uint64_t hi = lo + s.chunk < s.size ? lo + s.chunk : s.size;
+    // Store waiting keys: f(o) of ordinary bucket o, from line-buffer slot 0
+    // on, to the places p0(o)...  One item per (bucket, slot) over the
+    // segment's B/2 ordinary buckets, consecutive threads on consecutive slots
+    // of a bucket, so that a line's waiting part leaves in one store.
+    const uint32_t nitems = (B / 2) * LB;
+    auto store_waiting = [&](auto f_of, auto p0_of) {
+        for (uint32_t k = t; k < nitems; k += P) {
+            const uint32_t o = k / LB, q = k - o * LB;
+            const uint32_t f = f_of(o);
+            if (q < f) dst[p0_of(o, f) + q] = sm.lbuf[k];
+        }
+    };
+
+    struct Buf {
+        T x[E];
+        uint32_t b[E];
+    };
+    // a round at or past the end (the prefetch after the last round) reads
+    // key hi - 1 everywhere: in bounds, unconditional
+    auto load_round = [&](uint64_t base, Buf& R) {
+        const uint64_t bb = base < hi ? base : hi - 1;
+        const uint32_t cnt = (uint32_t)((hi - bb) < (uint64_t)ROUND ? (hi - bb) : (uint64_t)ROUND);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const uint32_t i = e * P + t;
+            const uint64_t ic = bb + (i < cnt ? i : cnt - 1);
+            R.x[e] = src[ic];
+            if constexpr (IDS) R.b[e] = isrc[ic];
+        }
+    };
+    auto do_round = [&](uint64_t base, Buf& cur, Buf& nxt) {
+        const uint32_t cnt = (uint32_t)((hi - base) < (uint64_t)ROUND ? (hi - base) : (uint64_t)ROUND);
+        load_round(base + ROUND, nxt);  // prefetch, unconditional so that no wait covers it
+        uint32_t bk[E], lr[E];
+        if constexpr (IDS) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) bk[e] = cur.b[e];
+        } else if (cnt == ROUND) {
+#pragma unroll
+            for (int e0 = 0; e0 < E; e0 += CLS_NB) {
+                K kx[CLS_NB];
+                uint32_t kb[CLS_NB];
+#pragma unroll
+                for (int u = 0; u < CLS_NB; ++u) kx[u] = Tr::enc(cur.x[e0 + u]);
+                classify_batch<K, CLS_NB>(sm.lsrt, sm.lguide, g, kx, kb);
+#pragma unroll
+                for (int u = 0; u < CLS_NB; ++u) bk[e0 + u] = kb[u];
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < E; ++e) bk[e] = classify(sm.lsrt, sm.lguide, g, Tr::enc(cur.x[e]));
+        }
+        if (cnt == ROUND) {
+            wave_rank_many<E>(sm.rh, sm.dummy, bk, lr);
+        } else {
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+                if (e * P + t < cnt) lr[e] = atomicAdd(&sm.rh[bk[e]], 1u);
+        }
+        __syncthreads();  // 1
+        // exclusive scan of rh: thread t owns buckets t*BPT .. t*BPT+BPT-1
+        uint32_t hv[BPT], v = 0;
+#pragma unroll
+        for (int k = 0; k < BPT; ++k) {
+            hv[k] = sm.rh[t * BPT + k];
+            v += hv[k];
+        }
+        uint32_t incl = v;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t y = __shfl_up(incl, d);
+            if (lane >= (uint32_t)d) incl += y;
+        }
+        if (lane == 63) sm.wtot[wid] = incl;
+        __syncthreads();  // 2
+        uint32_t woff = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) woff += (w < (int)wid) ? sm.wtot[w] : 0u;
+        uint32_t p = woff + incl - v;
+#pragma unroll
+        for (int k = 0; k < BPT; ++k) {
+            const uint32_t j = t * BPT + k;
+            const uint32_t c0 = sm.cursor[j], end = c0 + hv[k];
+            sm.rs[j] = (uint16_t)p;
+            sm.dbase[j] = c0 - p;
+            if (!(k & 1)) {
+                // places [c0 - w0, c0) wait (line aligned unless w0 == 0); after
+                // this round the places >= cut wait: the unfinished last line, or
+                // nothing while the run has not passed its chunk's first boundary
+                const uint32_t o = j >> 1, hd = sm.head[o], w0 = sm.nw[o];
+                const int64_t le = (int64_t)((dph + end) & ~(uint64_t)(LINE - 1)) - (int64_t)dph;
+                const uint32_t ct = end <= hd ? end : (le > (int64_t)hd ? (uint32_t)le : hd);
+                const bool done = w0 && ct > c0 - w0;  // their line is complete (else ct == c0 - w0)
+                sm.nfl[o] = (uint16_t)(done ? w0 : 0u);
+                sm.cut[o] = ct;
+                sm.nw[o] = (uint16_t)(end - ct);
+            }
+            sm.cursor[j] = end;
+            sm.rh[j] = 0;
+            p += hv[k];
+        }
+        __syncthreads();  // 3
+        // completed lines: their waiting keys first, at the places before the
+        // bucket's round start (old cursor = dbase + rs)
+        store_waiting([&](uint32_t o) { return (uint32_t)sm.nfl[o]; },
+                      [&](uint32_t o, uint32_t f) { return sm.dbase[2 * o] + sm.rs[2 * o] - f; });
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const uint32_t i = e * P + t;
+            if (i < cnt) {
+                const uint32_t q = sm.rs[bk[e]] + lr[e];
+                sm.stg[q] = cur.x[e];
+                sm.stb[q] = (uint16_t)bk[e];
+            }
+        }
+        __syncthreads();  // 4: staged, and the completed lines' waiting keys are read
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const uint32_t i = e * P + t;  // staged keys are grouped by bucket
+            if (i < cnt) {
+                const uint32_t bkt = sm.stb[i] & (BMAX - 1u);
+                const uint32_t d = sm.dbase[bkt] + i;
+                const T y = sm.stg[i];
+                bool wait = false;
+                if (!(bkt & 1u)) {
+                    const uint32_t ct = sm.cut[bkt >> 1];
+                    if (d >= ct) {  // d - ct < LB: the unfinished line holds fewer than LINE keys
+                        sm.lbuf[(bkt >> 1) * LB + (d - ct)] = y;
+                        wait = true;
+                    }
+                }
+                if (!wait) dst[d] = y;
+            }
+        }
+        // the next round reads cut/nw/lbuf only after its barriers 1-3, which
+        // every thread reaches after finishing the stores above
+    };
+    Buf A, Bb;
+    if (lo < hi) load_round(lo, A);
+    __syncthreads();
+    for (uint64_t base = lo; base < hi;) {
+        do_round(base, A, Bb);
+        base += ROUND;
+        if (base >= hi) break;
+        do_round(base, Bb, A);
+        base += ROUND;
+    }
+    __syncthreads();
+    // the chunk's unfinished last lines
+    store_waiting([&](uint32_t o) { return (uint32_t)sm.nw[o]; },
+                  [&](uint32_t o, uint32_t f) { return sm.cursor[2 * o] - f; });
+    __syncthreads();  // LDS is reloaded for the next chunk
+    }
+}
+
 // Scatter variants: the workgroup shape, and whether the bucket of each key
 // is read from the ids the count pass stored or classified again.
 template <class T, bool KV>
@@ -1201,6 +1440,18 @@ inline void launch_scatter(int shape, bool use_ids, uint32_t grid, hipStream_t s
 #define S512_SCATTER(I, PP, EE, W)                                                                              \
     k_scatter<T, KV, I, PP, EE, W><<<grid, PP, 0, st>>>(ps, cr, a, b, va, vb, trees, guides, hist, bstart, nchunk, \
                                                         ids)
+    if constexpr (!KV) {
+        if (shape == SCATTER_LB) {
+            // 4096-key rounds beside 8-byte keys (the line buffers take 60 KiB), 8192 beside 4-byte keys
+            constexpr int LE = sizeof(T) == 8 ? 8 : 16;
+            if (use_ids) k_scatter_lb<T, true, 512, LE><<<grid, 512, 0, st>>>(ps, cr, a, b, trees, guides, hist,
+                                                                                bstart, nchunk, ids);
+            else k_scatter_lb<T, false, 512, LE><<<grid, 512, 0, st>>>(ps, cr, a, b, trees, guides, hist, bstart,
+                                                                         nchunk, ids);
+            return;
+        }
+    }
+    if (shape == SCATTER_LB) shape = 0;  // key/value sorts: no line buffers beside the values
     if (use_ids) {
         switch (shape) {
             case 1: S512_SCATTER(true, 256, 16, false); break;

@@ -269,7 +269,11 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
         // slowest CU; 4x finer chunks measured 2-3% faster for 2^26..2^28
         // fp64 keys and 2^28 int32 keys, while smaller chunks (2^24 fp64,
         // 2^26 int32) measured 4-5% slower.
-        if (chunk * (sizeof(T) + (KV ? sizeof(int32_t) : 0)) >= (512u << 10)) chunk = (chunk + 3) / 4;
+        // The line-buffered scatter stores a (chunk, bucket) run's first and
+        // last line partially: it takes chunks of LB_MIN_CHUNK keys or more.
+        const bool lb = !KV && prm.scatter_shape == SCATTER_LB;
+        if (lb) chunk = std::max<uint64_t>(chunk, LB_MIN_CHUNK);
+        else if (chunk * (sizeof(T) + (KV ? sizeof(int32_t) : 0)) >= (512u << 10)) chunk = (chunk + 3) / 4;
         chunk = (chunk + SCAT_ROUND - 1) / SCAT_ROUND * SCAT_ROUND;
         chunk = std::max<uint64_t>(SCAT_ROUND, std::min<uint64_t>(chunk, prm.max_chunk));
         ps.resize(m);
@@ -705,8 +709,13 @@ EngineParams params_from(const s512_options* opt, bool wide) {
     if (opt->imbalance_pct) p.imbalance = (float)opt->imbalance_pct / 100.0f;
     if (opt->scatter_variant) {
         const uint32_t v = opt->scatter_variant - 1;
-        p.scatter_shape = (int)(v % SCATTER_SHAPES);
-        p.ids = v < SCATTER_SHAPES;
+        if (v >= 2 * SCATTER_SHAPES) {  // 9, 10: line-buffered, classified again / stored ids
+            p.scatter_shape = SCATTER_LB;
+            p.ids = v == 2 * SCATTER_SHAPES + 1;
+        } else {
+            p.scatter_shape = (int)(v % SCATTER_SHAPES);
+            p.ids = v < SCATTER_SHAPES;
+        }
     }
     if (opt->count_variant) p.count_shape = (int)opt->count_variant - 1;
     if (opt->tile_variant) p.tile_plan = (int)opt->tile_variant - 1;
@@ -788,8 +797,8 @@ static s512_status check_opts(const s512_options* opt) {
         set_error("count_variant %u out of range (max %d)", opt->count_variant, COUNT_SHAPES);
         return S512_ERR_INVALID;
     }
-    if (opt && opt->scatter_variant > 2 * SCATTER_SHAPES) {
-        set_error("scatter_variant %u out of range (max %d)", opt->scatter_variant, 2 * SCATTER_SHAPES);
+    if (opt && opt->scatter_variant > 2 * SCATTER_SHAPES + 2) {
+        set_error("scatter_variant %u out of range (max %d)", opt->scatter_variant, 2 * SCATTER_SHAPES + 2);
         return S512_ERR_INVALID;
     }
     if (opt && opt->target_leaf && opt->target_leaf < 64) {

@@ -215,7 +215,10 @@ typedef struct s512_options {
                                  stored ids; reclassifying runs it as shape 0), and
                                  reclassify = 1 classifies every key in the scatter again
                                  instead of reading the bucket ids that the count pass
-                                 stored (2 bytes per key)                             */
+                                 stored (2 bytes per key); 9 and 10: the line-buffered
+                                 scatter (every 128-byte line of a bucket's run stored
+                                 whole; keys-only sorts, key/value sorts run shape 0),
+                                 9 classifying again, 10 reading the stored ids         */
     uint32_t count_variant;   /* 0: default (shape 2); else 1 + count shape 0..4 = (threads,
                                  keys per thread per batch, keys classified together):
                                  256x16x8, 256x16x4, 256x8x8, 512x8x8, 256x8x4          */

@@ -40,12 +40,56 @@ def _run(s512, dev, a, **opts):
 
 @pytest.mark.parametrize("kind", ["normal", "distinct", "sorted"])
 @pytest.mark.parametrize("dtype", [np.int32, np.float64])
-@pytest.mark.parametrize("variant", range(1, 9))
+@pytest.mark.parametrize("variant", range(1, 11))
 def test_scatter_variants(s512, orc, dev, dtype, kind, variant):
     a = _input(orc, dtype, kind)
     got, st = _run(s512, dev, a, scatter_variant=variant)
     assert_keys_equal(got, orc.closed_sort(a.copy()), f"scatter_variant={variant} {kind}")
     assert st.levels >= 1
+
+
+@pytest.mark.parametrize("offset", [0, 1, 3])
+@pytest.mark.parametrize("dtype", [np.int32, np.float64])
+@pytest.mark.parametrize("variant", [9, 10])
+def test_line_buffered_scatter_unaligned(s512, orc, dev, dtype, variant, offset):
+    """The line-buffered scatter finds line boundaries from the destination's
+    address: arrays that start 1 or 3 keys past a 128-byte boundary (and the
+    scratch array, which is aligned) must sort the same; 9,437,187 keys give
+    many chunks per level, two levels and a ragged last round."""
+    import torch
+    n = 9_437_187
+    a = orc.gen_i32_full(n, 5) if dtype == np.int32 else orc.gen_f64("normal", n, 5)
+    base = torch.empty(n + 8, dtype=torch.int32 if dtype == np.int32 else torch.float64, device=dev)
+    t = base[offset:offset + n]
+    t.copy_(torch.from_numpy(a))
+    o = s512.Options()
+    o.scatter_variant = variant
+    st = s512.sort_ex(t, o)
+    assert st.levels == 2
+    assert_keys_equal(to_host(t), orc.closed_sort(a.copy()), f"scatter_variant={variant} offset={offset}")
+
+
+@pytest.mark.parametrize("variant", [9, 10])
+def test_line_buffered_scatter_wide_and_segmented(s512, dev, variant):
+    """int64 keys (8-byte lines of 16 keys) and a segmented batch through the
+    line-buffered scatter, against numpy's sort (the ascending order is
+    unique)."""
+    import torch
+    rng = np.random.default_rng(8)
+    k = rng.integers(-(1 << 62), 1 << 62, 2_500_003, dtype=np.int64)
+    t = to_dev(k, dev)
+    o = s512.Options()
+    o.scatter_variant = variant
+    s512.sort_ex(t, o)
+    assert np.array_equal(to_host(t), np.sort(k))
+    sizes = np.array([1_200_000, 3, 800_001, 0, 2_000_000], dtype=np.int64)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    a = rng.standard_normal(int(off[-1]))
+    ta = to_dev(a, dev)
+    toff = torch.from_numpy(off).to(dev)
+    s512.segsort(ta, toff, options=o)
+    want = np.concatenate([np.sort(a[off[i]:off[i + 1]]) for i in range(len(sizes))])
+    assert np.array_equal(to_host(ta).view(np.uint64), want.view(np.uint64))
 
 
 @pytest.mark.parametrize("kind", ["normal", "distinct"])
@@ -90,7 +134,7 @@ def test_fp64_minmax_tiles_with_variants(s512, orc, dev):
 def test_variant_out_of_range(s512, dev):
     import torch
     t = torch.zeros(10, dtype=torch.float64, device=dev)
-    for field, bad in (("scatter_variant", 9), ("count_variant", 6), ("tile_variant", 4)):
+    for field, bad in (("scatter_variant", 11), ("count_variant", 6), ("tile_variant", 4)):
         o = s512.Options()
         setattr(o, field, bad)
         with pytest.raises(s512.S512Error):
