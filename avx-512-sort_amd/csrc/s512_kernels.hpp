@@ -1204,7 +1204,13 @@ __global__ __launch_bounds__(P) void k_scatter(const PartSeg* ps, const ChunkRef
 // (chunk, bucket) run (shared with the previous chunk) and its last line at
 // the chunk's end are stored partially, so the planner gives this kernel
 // large chunks.  Equality buckets (odd) are stored directly.
-constexpr int SCATTER_LB = 4;                // scatter_shape of the line-buffered kernel
+// Measured slower than k_scatter on 2^28 keys, so it is opt-in
+// (scatter_variant 9, 10): fp64 2.07 vs 1.52 ms per launch (the 60 KiB of
+// line buffers leave room for 4096-key rounds only, which doubles the
+// per-round barriers and scans, and most keys pass through the line buffer,
+// two more LDS accesses each); int32 0.96 vs 0.85 ms, and 1.44 vs 0.79 ms on
+// 1000 distinct values.
+constexpr int SCATTER_LB = 4;               // scatter_shape of the line-buffered kernel
 constexpr uint64_t LB_MIN_CHUNK = 1u << 18;  // keys per chunk at least (partial lines ~2 per 1000)
 
 template <class T, bool IDS, int P, int E>
