@@ -93,6 +93,8 @@ def ref_lib():
     r.ref_kv_smallsort.argtypes = [_vp, _vp, _sz]
     r.ref_findmedian_i32.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp]
     r.ref_findmedian_f64.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp]
+    for name in ("ref_segsort_loop_i32", "ref_segsort_loop_f64"):
+        getattr(r, name).argtypes = [_vp, _vp, _sz, ctypes.c_int]
     return r
 
 

@@ -18,11 +18,28 @@
 //   sort512kv.hpp:5958 Partition512               ref_kv_partition
 //   sort512kv.hpp:5235 SmallSort16V               ref_kv_smallsort
 //   parallelInplace.hpp:78 FindMedian             ref_findmedian_{i32,f64}
+//   sort512.hpp:5960  Sort<T,size_t> per segment  ref_segsort_loop_{i32,f64}
 #include <cstddef>
 #include <cstdint>
 #include <omp.h>
 #include "sort512.hpp"
 #include "sort512kv.hpp"
+
+// A batch of independent arrays on the CPU: Sort512::Sort over every segment
+// [off[j], off[j+1]) of a, in order on one thread (parallel == 0) or spread
+// over the OpenMP threads with schedule(dynamic).  Segments of fewer than two
+// keys are skipped (the reference's Sort computes size-1).
+template <class T>
+static void segsort_loop(T* a, const uint64_t* off, size_t nseg, int parallel) {
+    if (parallel) {
+#pragma omp parallel for schedule(dynamic)
+        for (long j = 0; j < (long)nseg; ++j)
+            if (off[j + 1] - off[j] > 1) Sort512::Sort<T, size_t>(a + off[j], off[j + 1] - off[j]);
+    } else {
+        for (size_t j = 0; j < nseg; ++j)
+            if (off[j + 1] - off[j] > 1) Sort512::Sort<T, size_t>(a + off[j], off[j + 1] - off[j]);
+    }
+}
 
 extern "C" {
 void ref_sort_i32(int* a, size_t n) { Sort512::Sort<int, size_t>(a, n); }
@@ -46,4 +63,6 @@ void ref_kv_smallsort(int* k, int* v, size_t n) { Sort512kv::SmallSort16V(k, v, 
 void ref_findmedian_i32(int* a, int center, int size, int* mA, int* mB) { ParallelInplace::FindMedian(a, center, size, mA, mB); }
 void ref_findmedian_f64(double* a, int center, int size, int* mA, int* mB) { ParallelInplace::FindMedian(a, center, size, mA, mB); }
 int ref_max_threads(void) { return omp_get_max_threads(); }
+void ref_segsort_loop_i32(int* a, const uint64_t* off, size_t nseg, int parallel) { segsort_loop(a, off, nseg, parallel); }
+void ref_segsort_loop_f64(double* a, const uint64_t* off, size_t nseg, int parallel) { segsort_loop(a, off, nseg, parallel); }
 }
