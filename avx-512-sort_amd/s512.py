@@ -404,7 +404,7 @@ def partition(a, left, right, pivot):
 
 def partition_noalloc(a, left, right, pivot):
     """Partition512 of a float64 device tensor with the library's fixed
-    workspace (no allocation; at most about 8.3 million keys per call)."""
+    workspace (no allocation; at most 8,384,480 keys per call)."""
     out = _c_size()
     _check(lib.s512_partition_f64_dev_noalloc(_ptr(a), left, right, float(pivot), ctypes.byref(out), _stream_of(a)))
     return out.value

@@ -307,6 +307,66 @@ def test_partition_live_reference(orc, ref):
         assert np.array_equal(np.sort(b1), np.sort(b2))
 
 
+def test_kv_partition_live_reference(orc, ref):
+    """orc_partition_kv against the compiled Sort512kv::Partition512
+    (sort512kv.hpp:5958): the same bound, the same (key, value) multiset in
+    each side, and the closed form (bound = left + #{keys <= pivot}); values
+    are key*100+1 so that a pair split apart fails."""
+    rng = np.random.default_rng(12)
+    for _ in range(300):
+        n = int(rng.integers(1, 3000))
+        k = rng.integers(-1000, 1000, n).astype(np.int32)
+        v = (k.astype(np.int64) * 100 + 1).astype(np.int32)
+        l = int(rng.integers(0, n))
+        r = int(rng.integers(l, n))
+        pivot = int(rng.integers(-1100, 1100))
+        k1, v1, k2, v2 = k.copy(), v.copy(), k.copy(), v.copy()
+        got = ref.ref_kv_partition(_p(k1), _p(v1), l, r, pivot)
+        b = orc.kv_partition(k2, v2, l, r, pivot)
+        assert b == got == l + int(np.count_nonzero(k[l:r + 1] <= pivot))
+        assert np.all(k2[l:b] <= pivot) and np.all(k2[b:r + 1] > pivot)
+        assert np.array_equal(k2[:l], k[:l]) and np.array_equal(k2[r + 1:], k[r + 1:])
+        assert np.array_equal(v2, (k2.astype(np.int64) * 100 + 1).astype(np.int32))
+        for lo, hi in ((l, b), (b, r + 1)):
+            assert np.array_equal(orc.pair_multiset(k1[lo:hi], v1[lo:hi]), orc.pair_multiset(k2[lo:hi], v2[lo:hi]))
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.float64])
+def test_segsort_matches_numpy_per_segment(orc, dtype):
+    """orc_segsort (the caller's loop of Sort512::Sort, sort512.hpp:5960)
+    against numpy's sort of each segment (the ascending order is unique):
+    empty, one-key, leaf-sized and multi-level segments; decreasing offsets
+    are refused."""
+    rng = np.random.default_rng(13)
+    sizes = np.array([0, 1, 2, 3, 17, 128, 129, 256, 257, 0, 5000, 70_000, 1, 300_001], dtype=np.int64)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    n = int(off[-1])
+    a = rng.integers(-(1 << 31), 1 << 31, n).astype(np.int32) if dtype == np.int32 else rng.standard_normal(n)
+    a[70_000:70_100] = a[70_000]  # repeated keys inside a segment
+    got = orc.segsort(a.copy(), off)
+    want = np.concatenate([np.sort(a[off[i]:off[i + 1]]) for i in range(len(sizes))])
+    assert np.array_equal(got.view(np.uint64 if dtype == np.float64 else np.uint32),
+                          want.view(np.uint64 if dtype == np.float64 else np.uint32))
+    bad = off.copy()
+    bad[3], bad[4] = bad[4], bad[3]
+    with pytest.raises(AssertionError):
+        orc.segsort(a.copy(), bad)
+
+
+def test_segsort_live_reference_loop(orc, ref):
+    """orc_segsort against the compiled reference's Sort looped over the same
+    segments, on one thread and under OpenMP."""
+    rng = np.random.default_rng(14)
+    sizes = rng.integers(0, 3000, 400).astype(np.int64)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    a = rng.standard_normal(int(off[-1]))
+    want = orc.segsort(a.copy(), off)
+    for par in (0, 1):
+        b = a.copy()
+        ref.ref_segsort_loop_f64(_p(b), _p(off), len(sizes), par)
+        assert np.array_equal(b.view(np.uint64), want.view(np.uint64)), par
+
+
 # ---------------------------------------------------------------- MLP
 def test_mlp_gradient_finite_differences(orc):
     """Central differences on the fp64 restatement of the model gradient
