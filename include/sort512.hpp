@@ -25,6 +25,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <type_traits>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #include "s512.h"
 
@@ -119,8 +122,9 @@ inline void SmallSort16V(int* __restrict__ ptr, const size_t length) {
 }  // namespace Sort512
 
 namespace ParallelInplace {
-// The reference's version is called by every thread of an OpenMP team with
-// thread bookkeeping arguments; the GPU merge is one call per merge.
+// Merges the sorted runs [0, centerPosition) and [centerPosition, sizeArray)
+// of array in place: one GPU merge per call (no reference counterpart of this
+// three-argument form; the reference's form follows).
 template <class NumType>
 inline void parallelMergeInPlace(NumType array[], const int sizeArray, int centerPosition) {
     static_assert(std::is_same<NumType, int>::value || std::is_same<NumType, double>::value,
@@ -131,6 +135,41 @@ inline void parallelMergeInPlace(NumType array[], const int sizeArray, int cente
     else
         st = s512_merge_inplace_f64(reinterpret_cast<double*>(array), (size_t)sizeArray, (size_t)centerPosition);
     Sort512::detail::check(st, "parallelMergeInPlace");
+}
+
+// The reference's bookkeeping record (parallelInplace.hpp:138-145), kept so
+// that callers declaring arrays of it compile unchanged.
+template <class NumType>
+struct WorkingInterval {
+    NumType* array;
+    int currentStart;
+    int currentMiddle;
+    int currentEnd;
+    int level;
+    int depthLimite;
+};
+
+// parallelInplace.hpp:209-211: called by each of the numThreadsInvolved
+// threads from firstThread of an OpenMP team, with the team's shared
+// intervals and barrier arrays.  A GPU merge is one call: the first involved
+// thread runs it and returns once the merge is complete; the other involved
+// threads have no share of the work and return at once (in the reference
+// every involved thread returns after the merge).  Callers that publish a
+// merge's completion from its first thread, as the reference's
+// SortOmpParMerge does (sort512.hpp:6160-6166), work unchanged.  intervals
+// and barrier are not touched.
+template <class NumType>
+inline void parallelMergeInPlace(NumType array[], const int sizeArray, int centerPosition,
+                                 const long int numThreadsInvolved, const long int firstThread,
+                                 volatile WorkingInterval<NumType> intervals[], volatile int barrier[]) {
+    (void)numThreadsInvolved;
+    (void)intervals;
+    (void)barrier;
+    long int me = 0;
+#ifdef _OPENMP
+    me = omp_get_thread_num();
+#endif
+    if (me == firstThread) parallelMergeInPlace(array, sizeArray, centerPosition);
 }
 }  // namespace ParallelInplace
 

@@ -127,7 +127,58 @@ static void test_merge() {
         }
 }
 
+// The reference's seven-argument form, called the way its SortOmpParMerge
+// calls it (sort512.hpp:6104-6174): T threads sort T chunks, then at each
+// level every thread of a group of 2^level calls the merge with the group's
+// first thread, which then publishes that the group's run is done; a group's
+// first thread first waits for the other half's flag.
+static void test_merge_team() {
+#ifdef _OPENMP
+    const int T = 4;
+    for (int n : {7, 1000, 100003, 2000000}) {
+        std::vector<int> a(n);
+        rand_vec(a);
+        std::vector<int> want = a;
+        std::sort(want.begin(), want.end());
+        ParallelInplace::WorkingInterval<int> intervals[T] = {};
+        int barrier[T] = {0};
+        int done[3][T] = {{0}};
+        const int chunk = (n + T - 1) / T;
+#pragma omp parallel num_threads(T)
+        {
+            const int me = omp_get_thread_num();
+            const int first = std::min(n, chunk * me), last = std::min(n, chunk * (me + 1));
+            std::sort(a.begin() + first, a.begin() + last);
+#pragma omp atomic write
+            done[0][me] = 1;
+            for (int level = 1; (1 << level) <= T; ++level) {
+                const int lead = (me >> level) << level;
+                const int half = (me >> (level - 1)) << (level - 1);
+                const int other = (half >> (level - 1)) + (lead == half ? 1 : -1);
+                for (;;) {
+                    int d;
+#pragma omp atomic read
+                    d = done[level - 1][other];
+                    if (d) break;
+                }
+                const int lo = std::min(n, lead * chunk);
+                const int mid = std::min(n, lo + (1 << (level - 1)) * chunk);
+                const int hi = std::min(n, lo + (1 << level) * chunk);
+                ParallelInplace::parallelMergeInPlace(&a[lo], hi - lo, mid - lo, 1L << level, (long)lead, intervals,
+                                                      barrier);
+                if (me == lead) {
+#pragma omp atomic write
+                    done[level][me >> level] = 1;
+                }
+            }
+        }
+        EXPECT(a == want, "team merge n=%d", n);
+    }
+#endif
+}
+
 int main() {
+    test_merge_team();
     test_sorts<int>();
     test_sorts<double>();
     test_partition<int>();
