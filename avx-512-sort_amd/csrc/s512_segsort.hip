@@ -18,9 +18,17 @@
 //   one partition level over all large segments: sample, sample sort,
 //   splitter tree, count, column scan, scatter, plan (the plan appends the
 //   buckets to the same tile lists as the small segments)
+//   a second partition level over the buckets the first level's plan left
+//   on the "next" list (still larger than the largest tile): k_seg_reset_level
+//   clears the level counters and the space pools (the first level's
+//   splitters, histograms and bucket tables are dead by then), k_seg_level2
+//   turns every such bucket into a partition descriptor, and the same eight
+//   kernels run again; their plan appends to the same tile lists
 //   copy-back of equality buckets, then the tile sorts of every class
-// A bucket still larger than the largest tile lands on the "next" list; the
-// host finishes those with the multi-level engine after the sequence.
+// A bucket still larger than the largest tile after two levels lands on a
+// second next list; the host finishes those with the multi-level engine after
+// the sequence (and every bucket of the first next list if the second level's
+// descriptors do not fit the pools).
 #include "s512_engine_internal.hpp"
 #include "s512_tile_f64.hpp"
 #include "s512_hostio.hpp"
@@ -40,7 +48,8 @@ constexpr int NTOP = 5;                // space pools: hist, splitters, buckets,
 // class u64 [40..55] (indices in u64 words).
 constexpr int CTL_WORDS64 = 56;
 constexpr int CNT_NEXT = 9, CNT_COPY = 8, CNT_NBIG = 10, CNT_NCHUNK = 11, CNT_OVERFLOW = 12, CNT_SPECIAL = 13,
-              CNT_PLAN_OVERFLOW = 14;
+              CNT_PLAN_OVERFLOW = 14, CNT_NEXT2 = 15, CNT_OVERFLOW2 = 16, CNT_NBIG2 = 17,
+              CNT_NBIG1 = 18;
 
 struct SegClassify {
     PlanLists pl;
@@ -165,6 +174,82 @@ __global__ void k_seg_guard(uint32_t* counters) {
     if (counters[CNT_OVERFLOW] && threadIdx.x < 12) counters[threadIdx.x] = 0;
 }
 
+// Between the two levels: the partition counters and the space pools start
+// again from zero (the first level's descriptors are no longer read).
+__global__ void k_seg_reset_level(uint64_t* ctl) {
+    uint32_t* cnt = (uint32_t*)ctl;
+    if (threadIdx.x < NTOP) ctl[32 + threadIdx.x] = 0;
+    if (threadIdx.x == 0) {
+        cnt[CNT_NBIG1] = cnt[CNT_NBIG];  // kept for the statistics
+        cnt[CNT_NBIG] = 0;
+        cnt[CNT_NCHUNK] = 0;
+    }
+}
+
+// Second level: one partition descriptor per bucket on the first next list,
+// from the same pools as the first level's (one thread per bucket; global
+// atomics, the list is short).  Keys of such a bucket are in the scratch
+// array (buf = 1), so its partition writes them back to the caller's array.
+__global__ __launch_bounds__(256) void k_seg_level2(const Seg* next, const uint32_t* nnext, SegClassify c,
+                                                    uint32_t* counters) {
+    const uint32_t n = *nnext;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const Seg sg = next[i];
+        int lf = ceil_log2_dev((sg.size + c.target_leaf - 1) / c.target_leaf);
+        lf = lf < 1 ? 1 : (lf > (int)c.max_logf ? (int)c.max_logf : lf);
+        const uint32_t F = 1u << lf, B = 2 * F;
+        const uint32_t nch = (uint32_t)((sg.size + SEG_CHUNK - 1) / SEG_CHUNK);
+        const uint32_t nsamp = c.oversample * F < SEG_SMAX ? c.oversample * F : SEG_SMAX;
+        const unsigned long long need[NTOP] = {(unsigned long long)nch * B, F, B,
+                                               (unsigned long long)guide_words(F), nsamp};
+        const uint32_t o = atomicAdd(c.nbig, 1u);
+        const uint32_t cb = atomicAdd(c.nchunk, nch);
+        unsigned long long rt[NTOP];
+        bool fits = o < c.nbig_cap && (uint64_t)cb + nch <= c.nchunk_cap;
+#pragma unroll
+        for (int q = 0; q < NTOP; ++q) {
+            rt[q] = atomicAdd(&c.tops[q], need[q]);
+            fits = fits && rt[q] + need[q] <= c.top_cap[q];
+        }
+        if (!fits) {
+            atomicOr(&counters[CNT_OVERFLOW2], 1u);
+            continue;
+        }
+        PartSeg p;
+        p.begin = sg.begin;
+        p.size = sg.size;
+        p.buf = sg.buf;
+        p.logf = (uint32_t)lf;
+        p.nchunks = nch;
+        p.nsample = nsamp;
+        p.chunk = SEG_CHUNK;
+        p.hist_base = rt[0];
+        p.tree_base = rt[1];
+        p.bkt_base = rt[2];
+        p.guide_base = rt[3];
+        p.kmin = 0;
+        p.shift = 0;
+        p.mode = 0;
+        p.lshift = 0;
+        p.lbase = 0;
+        p.flags = 0;
+        p.sample_base = rt[4];
+        c.ps[o] = p;
+        c.sl[o] = TileSeg{p.sample_base, nsamp, 0u};
+        for (uint32_t k = 0; k < nch; ++k) c.cr[cb + k] = ChunkRef{o, k};
+        atomicAdd(&counters[CNT_NBIG2], 1u);
+    }
+}
+
+// A second level whose descriptors did not all fit runs on nothing: the host
+// finishes the whole first next list instead.
+__global__ void k_seg_guard2(uint32_t* counters) {
+    if (counters[CNT_OVERFLOW2] && threadIdx.x == 0) {
+        counters[CNT_NBIG] = 0;
+        counters[CNT_NCHUNK] = 0;
+    }
+}
+
 // ---------------------------------------------------------------- workspace
 // Device workspace of one segmented sort, carved out of one allocation.
 struct SegWs {
@@ -182,19 +267,24 @@ struct SegWs {
     uint64_t* bsize;
     TileSeg* tile[8];
     TileSeg* copy;
-    Seg* next;
+    Seg* next;   // first level's buckets still larger than the largest tile
+    Seg* next2;  // the second level's
     uint32_t next_cap;
     SegClassify cls;
+    PlanLists pl2;  // the second level's plan: as cls.pl, appending to next2
     size_t bytes;
 };
 
-// Capacities are upper bounds, so no list can overflow:
+// Capacities are upper bounds, so no list can overflow (per level: the
+// second level's segments are disjoint key ranges larger than a tile too):
 //   large segments (> largest tile)   nbig   <= nkeys / (maxcap + 1)
 //   chunks                                   <= nkeys / SEG_CHUNK + nbig
 //   ordinary buckets of one segment   F = 2^ceil(log2(ceil(size / leaf))) < 2 size / leaf + 2,
 //                                     summed <= 2 floor(nkeys / leaf) + 2 nbig + 2 = obkt
-//   tile list k                              <= segments + min(obkt, nkeys / (cap[k-1] + 1))
-//   copy pieces (equality buckets and single keys, F of each per segment)
+//   tile list k (segments, then the buckets of both levels)
+//                                            <= segments + min(2 obkt, nkeys / (cap[k-1] + 1))
+//   copy pieces (equality buckets and single keys of the first level, F of
+//   each per segment; the second level writes to the caller's array)
 //                                            <= 2 obkt + nkeys / COPY_PIECE
 template <class T>
 static SegWs seg_layout(char* base, size_t nseg, size_t nkeys, const EngineParams& prm, int ncls,
@@ -203,7 +293,7 @@ static SegWs seg_layout(char* base, size_t nseg, size_t nkeys, const EngineParam
     SegWs w;
     memset(&w, 0, sizeof(w));
     const uint32_t maxcap = caps[ncls - 1];
-    const uint64_t nbig_cap = std::max<uint64_t>(1, std::min<uint64_t>(nseg, nkeys / (maxcap + 1)));
+    const uint64_t nbig_cap = std::max<uint64_t>(1, nkeys / (maxcap + 1));
     const uint64_t nchunk_cap = nkeys / SEG_CHUNK + nbig_cap;
     const uint64_t obkt = std::min<uint64_t>(nbig_cap * FMAX, 2 * (nkeys / prm.target_leaf) + 2 * nbig_cap + 2);
     const uint64_t s_cap = std::min<uint64_t>(nbig_cap * SEG_SMAX, (uint64_t)prm.oversample * obkt);
@@ -221,7 +311,7 @@ static SegWs seg_layout(char* base, size_t nseg, size_t nkeys, const EngineParam
     w.bsize = ar.take<uint64_t>(2 * obkt);
     SegClassify& c = w.cls;
     for (int k = 0; k < ncls; ++k) {
-        const uint64_t from_buckets = std::min<uint64_t>(obkt, nkeys / (k ? caps[k - 1] + 1 : 2));
+        const uint64_t from_buckets = std::min<uint64_t>(2 * obkt, nkeys / (k ? caps[k - 1] + 1 : 2));
         const uint64_t cap = std::min<uint64_t>(nseg + from_buckets, 0xFFFFFFFFull);
         w.tile[k] = ar.take<TileSeg>(cap);
         c.list_cap[k] = (uint32_t)cap;
@@ -232,6 +322,7 @@ static SegWs seg_layout(char* base, size_t nseg, size_t nkeys, const EngineParam
     w.copy = ar.take<TileSeg>(copy_cap);
     w.next_cap = (uint32_t)std::min<uint64_t>(obkt, 0xFFFFFFFFull);
     w.next = ar.take<Seg>(w.next_cap);
+    w.next2 = ar.take<Seg>(w.next_cap);
     w.bytes = ar.off;
     c.pl.nclass = (uint32_t)ncls;
     c.pl.tile_count = w.counters;
@@ -263,6 +354,9 @@ static SegWs seg_layout(char* base, size_t nseg, size_t nkeys, const EngineParam
     c.max_logf = (uint32_t)prm.max_logf;
     c.maxcap = maxcap;
     c.nkeys = nkeys;
+    w.pl2 = c.pl;
+    w.pl2.next = w.next2;
+    w.pl2.next_count = w.counters + CNT_NEXT2;
     return w;
 }
 
@@ -284,36 +378,51 @@ static s512_status seg_enqueue(T* a, T* b, const uint64_t* off, size_t nseg, con
     S512_CHECK_LAUNCH();
     const uint32_t* nbig = w.counters + CNT_NBIG;
     const uint32_t* nchunk = w.counters + CNT_NCHUNK;
-    k_sample<T><<<grid_of(nbig_cap, 2048), 256, 0, st>>>(w.ps, nbig, a, b, (K*)w.samples, SEG_SMAX, prm.seed);
-    S512_CHECK_LAUNCH();
-    // each sample by the smallest of four networks (1024..8192 keys) that holds it
-    {
-        const uint32_t g = grid_of(nbig_cap, 2048);
-        K* smp = (K*)w.samples;
-        k_tile_sort_range<K, 512, 2><<<g, 512, 0, st>>>(w.sl, nbig, smp, smp, 0u);
+    // one partition level over the descriptors in w.ps (count in nbig)
+    auto level = [&](const PlanLists& pl) -> s512_status {
+        k_sample<T><<<grid_of(nbig_cap, 2048), 256, 0, st>>>(w.ps, nbig, a, b, (K*)w.samples, SEG_SMAX, prm.seed);
         S512_CHECK_LAUNCH();
-        k_tile_sort_range<K, 512, 4><<<g, 512, 0, st>>>(w.sl, nbig, smp, smp, 1024u);
+        // each sample by the smallest of four networks (1024..8192 keys) that holds it
+        {
+            const uint32_t g = grid_of(nbig_cap, 2048);
+            K* smp = (K*)w.samples;
+            k_tile_sort_range<K, 512, 2><<<g, 512, 0, st>>>(w.sl, nbig, smp, smp, 0u);
+            S512_CHECK_LAUNCH();
+            k_tile_sort_range<K, 512, 4><<<g, 512, 0, st>>>(w.sl, nbig, smp, smp, 1024u);
+            S512_CHECK_LAUNCH();
+            k_tile_sort_range<K, 512, 8><<<g, 512, 0, st>>>(w.sl, nbig, smp, smp, 2048u);
+            S512_CHECK_LAUNCH();
+            k_tile_sort_range<K, 512, 16><<<g, 512, 0, st>>>(w.sl, nbig, smp, smp, 4096u);
+        }
         S512_CHECK_LAUNCH();
-        k_tile_sort_range<K, 512, 8><<<g, 512, 0, st>>>(w.sl, nbig, smp, smp, 2048u);
+        k_build_tree<K><<<grid_of(nbig_cap, 2048), FMAX, 0, st>>>(w.ps, nbig, (K*)w.samples, SEG_SMAX, (K*)w.trees,
+                                                                  w.guides);
         S512_CHECK_LAUNCH();
-        k_tile_sort_range<K, 512, 16><<<g, 512, 0, st>>>(w.sl, nbig, smp, smp, 4096u);
-    }
+        launch_count<T>(prm.count_shape, grid_of(nchunk_cap, 8192), st, w.ps, w.cr, nchunk, a, b, (K*)w.trees,
+                        w.guides, w.hist, w.counters + CNT_SPECIAL, nullptr);
+        S512_CHECK_LAUNCH();
+        k_colscan<4><<<grid_of((uint64_t)nbig_cap * CS_NSLICE, 8192), 256, 0, st>>>(w.ps, nbig, w.hist, w.bsize);
+        S512_CHECK_LAUNCH();
+        k_bucket_scan<<<grid_of(nbig_cap, 2048), BMAX, 0, st>>>(w.ps, nbig, w.bsize, w.bstart);
+        S512_CHECK_LAUNCH();
+        launch_scatter<T, false>(prm.scatter_shape, false, grid_of(nchunk_cap, 4096), st, w.ps, w.cr, a, b, nullptr,
+                                 nullptr, (K*)w.trees, w.guides, w.hist, w.bstart, nchunk, nullptr);
+        S512_CHECK_LAUNCH();
+        k_plan<<<dim3(grid_of(nbig_cap, 2048), BMAX / 256), 256, 0, st>>>(w.ps, nbig, w.bstart, w.bsize, pl);
+        S512_CHECK_LAUNCH();
+        return S512_OK;
+    };
+    s512_status rc = level(w.cls.pl);
+    if (rc) return rc;
+    // the second level, over the first level's buckets larger than a tile
+    k_seg_reset_level<<<1, 64, 0, st>>>(w.ctl);
     S512_CHECK_LAUNCH();
-    k_build_tree<K><<<grid_of(nbig_cap, 2048), FMAX, 0, st>>>(w.ps, nbig, (K*)w.samples, SEG_SMAX, (K*)w.trees,
-                                                              w.guides);
+    k_seg_level2<<<grid_of((w.next_cap + 255) / 256, 256), 256, 0, st>>>(w.next, w.counters + CNT_NEXT, w.cls,
+                                                                          w.counters);
     S512_CHECK_LAUNCH();
-    launch_count<T>(prm.count_shape, grid_of(nchunk_cap, 8192), st, w.ps, w.cr, nchunk, a, b, (K*)w.trees, w.guides,
-                    w.hist, w.counters + CNT_SPECIAL, nullptr);
+    k_seg_guard2<<<1, 64, 0, st>>>(w.counters);
     S512_CHECK_LAUNCH();
-    k_colscan<4><<<grid_of((uint64_t)nbig_cap * CS_NSLICE, 8192), 256, 0, st>>>(w.ps, nbig, w.hist, w.bsize);
-    S512_CHECK_LAUNCH();
-    k_bucket_scan<<<grid_of(nbig_cap, 2048), BMAX, 0, st>>>(w.ps, nbig, w.bsize, w.bstart);
-    S512_CHECK_LAUNCH();
-    launch_scatter<T, false>(prm.scatter_shape, false, grid_of(nchunk_cap, 4096), st, w.ps, w.cr, a, b, nullptr,
-                             nullptr, (K*)w.trees, w.guides, w.hist, w.bstart, nchunk, nullptr);
-    S512_CHECK_LAUNCH();
-    k_plan<<<dim3(grid_of(nbig_cap, 2048), BMAX / 256), 256, 0, st>>>(w.ps, nbig, w.bstart, w.bsize, w.cls.pl);
-    S512_CHECK_LAUNCH();
+    if ((rc = level(w.pl2))) return rc;
     k_copy_back<T, false><<<2048, 256, 0, st>>>(w.copy, w.counters + CNT_COPY, a, b, nullptr, nullptr);
     S512_CHECK_LAUNCH();
     for (uint32_t k = 0; k < w.cls.pl.nclass; ++k) {
@@ -339,9 +448,13 @@ static s512_status seg_finish(T* a, T* b, const uint64_t* off, size_t nseg, size
     S512_HIP(hipMemcpyAsync(h, w.ctl, CTL_WORDS64 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
     S512_HIP(hipStreamSynchronize(st));
     const uint32_t* hc = (const uint32_t*)h;
-    const uint32_t overflow = hc[CNT_OVERFLOW], nnext = hc[CNT_NEXT], nbig = hc[CNT_NBIG];
+    // after a completed second level the host finishes only the second next
+    // list; if the second level did not fit, the whole first one
+    const bool l2 = hc[CNT_OVERFLOW2] == 0;
+    const uint32_t overflow = hc[CNT_OVERFLOW], nnext = l2 ? hc[CNT_NEXT2] : hc[CNT_NEXT];
+    const Seg* dnext = l2 ? w.next2 : w.next;
     if (stats) {
-        stats->levels = nbig ? 1 : 0;
+        stats->levels = hc[CNT_NBIG1] ? (hc[CNT_NBIG2] && l2 ? 2 : 1) : 0;
         for (uint32_t k = 0; k < w.cls.pl.nclass; ++k) {
             stats->tiles += hc[k];
             stats->keys[S512_PHASE_TILE] += h[40 + k];
@@ -370,7 +483,7 @@ static s512_status seg_finish(T* a, T* b, const uint64_t* off, size_t nseg, size
         }
         if ((rc = ds.pinned2.reserve((size_t)nnext * sizeof(Seg), st))) return rc;
         Seg* hs = (Seg*)ds.pinned2.ptr;
-        S512_HIP(hipMemcpyAsync(hs, w.next, (size_t)nnext * sizeof(Seg), hipMemcpyDeviceToHost, st));
+        S512_HIP(hipMemcpyAsync(hs, dnext, (size_t)nnext * sizeof(Seg), hipMemcpyDeviceToHost, st));
         S512_HIP(hipStreamSynchronize(st));
         segs.assign(hs, hs + nnext);
     }

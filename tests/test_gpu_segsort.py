@@ -69,18 +69,35 @@ def test_segsort_edge_sizes(s512, orc, dev, dtype):
     _check_against_oracle(orc, got, a0, off, "edge sizes")
 
 
-def test_segsort_oversized_buckets_finish_on_host_engine(s512, orc, dev):
-    """One 6M-key fp64 segment: the single partition level (fan-out <= 512)
-    leaves buckets above the largest tile; they finish in the multi-level
-    engine and the result is still exact."""
+def test_segsort_oversized_buckets_second_level(s512, orc, dev):
+    """One 6M-key fp64 segment: the first partition level leaves buckets
+    above the largest tile; the sequence's second level finishes them, so no
+    bucket goes to the host engine, and the result is exact."""
     import torch
     sizes = [5, 6_000_000, 1000, 9000]
     off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
     t = _keys(orc, s512, dev, np.float64, int(off[-1]), seed=21, kind="uniform")
     a0 = to_host(t)
     st = s512.segsort(t, torch.from_numpy(off.view(np.int64)).to(dev), stats=True)
-    assert st.learned_levels > 0, st.as_dict()
+    assert st.levels == 2 and st.learned_levels == 0, st.as_dict()
     _check_against_oracle(orc, to_host(t), a0, off, "oversized buckets")
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.float64])
+def test_segsort_oversized_after_two_levels_finish_on_host_engine(s512, orc, dev, dtype):
+    """4-way levels (max_fanout_log2 = 2) leave buckets above the largest
+    tile even after the second level: those finish in the multi-level engine
+    after the sequence, and the result is still exact."""
+    import torch
+    sizes = [3, 2_000_003, 40_000, 0, 700_001]
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    t = _keys(orc, s512, dev, dtype, int(off[-1]), seed=22)
+    a0 = to_host(t)
+    o = s512.Options()
+    o.max_fanout_log2 = 2
+    st = s512.segsort(t, torch.from_numpy(off.view(np.int64)).to(dev), stats=True, options=o)
+    assert st.levels == 2 and st.learned_levels > 0, st.as_dict()
+    _check_against_oracle(orc, to_host(t), a0, off, "oversized after two levels")
 
 
 def test_segsort_special_values(s512, orc, dev):
@@ -200,3 +217,4 @@ def test_segsort_full_workload(s512, orc, dev, dtype, capsys):
     with capsys.disabled():
         print(f"\nsegsort 200k lognormal {np.dtype(dtype).name}: {n} keys, {dt * 1e3:.2f} ms "
               f"({n / dt / 1e9:.2f} G keys/s incl. host sync), engine buckets={st.learned_levels}")
+    assert st.learned_levels == 0, "every bucket finishes inside the sequence"
