@@ -1190,6 +1190,290 @@ __global__ __launch_bounds__(P) void k_scatter(const PartSeg* ps, const ChunkRef
     }
 }
 
+// ---------------------------------------------------------------- full-line scatter
+// Keys only, reading the bucket ids that the count pass stored.  Measured on
+// MI355X (2^28 fp64 keys written as 512 streams per workgroup): a 128-byte
+// line stored whole by one store instruction streams at 5.7 TB/s; the same
+// bytes as runs of 16 keys off line alignment -- every line stored in two
+// parts, one per round, the pattern of k_scatter -- at 1.9 TB/s, at 3.5 TB/s
+// when the parts are 64-byte halves, and neither cache hints nor storing the
+// two parts close together in time changes that.
+//
+// So every store here covers whole lines.  A bucket's keys that do not
+// complete their line in a round wait in LDS (`carry`, at most LINE - 1 keys
+// per ordinary bucket) and go out at the front of the bucket's next run.  The
+// store phase works on line jobs: LINE consecutive lanes write one line,
+// reading each key from the carry or from the staging area.
+//
+// A workgroup takes a contiguous range of chunks (a persistent grid of about
+// one workgroup per CU).  The chunks of one segment inside that range are one
+// stream: bucket j's destination continues from chunk to chunk, so only the
+// stream's first line of each bucket (shared with the previous workgroup's
+// range) and its last line are stored partially.  Equality buckets and the
+// stream's last round store everything.
+//
+// Per round (the next round's keys and ids are loaded meanwhile):
+//   rank every key inside the round with LDS atomics          | barrier 1
+//   bucket owners: the last round's waiting keys into the carry; this round's
+//   writes (carried keys first, then staged ones, up to the last line
+//   boundary), staging start and line jobs; a packed block scan of
+//   (keys, jobs)                                              | barrier 2, 3
+//   stage the keys grouped by bucket                          | barrier 4
+//   line jobs: LINE lanes per line, one store instruction per 64 / LINE lines
+constexpr int SCATTER_FL = 5;  // scatter_shape of the full-line kernel
+
+template <class T, int P, int E>
+struct ScatterFlLds {
+    static constexpr int ROUND = P * E;
+    static constexpr int LINE = 128 / (int)sizeof(T);  // keys per 128-byte line
+    static constexpr int CW = LINE - 1;                // waiting keys per ordinary bucket, at most
+    // line jobs of a round: every staged and carried key in full lines plus a
+    // partial first and last line per bucket
+    static constexpr int JMAX = (ROUND + FMAX * CW) / LINE + 2 * BMAX;
+    struct alignas(16) Rec {
+        uint32_t d;   // first place written this round (segment-relative)
+        uint16_t w;   // keys written: the k carried ones, then staged ones
+        uint16_t rs;  // the bucket's staged keys: stg[rs ...]
+        uint16_t js;  // its first line job
+        uint16_t ts;  // waiting keys after this round: stg[ts .. ts + tn) ...
+        uint8_t k;    // carried keys written first (ordinary buckets)
+        uint8_t a;    // line offset of d (nonzero only on a stream's first line)
+        uint8_t tn;
+        uint8_t cd;   // ... go to carry slots cd .. cd + tn
+    };
+    static_assert(sizeof(Rec) == 16, "record layout");
+    Rec rec[BMAX];
+    uint32_t cursor[BMAX];  // next place of each bucket (segment-relative, carried keys included)
+    uint32_t rh[BMAX];      // round histogram
+    uint16_t jb[JMAX];      // bucket of each line job
+    uint32_t wtot[P / 64];
+    uint32_t dummy[DUMMY_SLOTS];
+    T stg[ROUND];
+    T carry[FMAX * CW];
+};
+
+template <class T, int P, int E>
+__global__ __launch_bounds__(P) void k_scatter_fl(const PartSeg* ps, const ChunkRef* cr, T* a, T* b,
+                                                  const uint32_t* hist, const uint64_t* bstart,
+                                                  const uint32_t* nchunk, const uint16_t* ids, uint32_t per_wg) {
+    using L = ScatterFlLds<T, P, E>;
+    using Rec = typename L::Rec;
+    constexpr int ROUND = L::ROUND, LINE = L::LINE, CW = L::CW;
+    constexpr int NW = P / 64;
+    constexpr int JPI = 64 / LINE;  // line jobs per wave store instruction
+    static_assert(BMAX == 2 * P, "one ordinary and one equality bucket per thread");
+    static_assert(ROUND < 65536 && (LINE & (LINE - 1)) == 0 && LINE <= 64, "full-line scatter shape");
+    __shared__ L sm;
+    const uint32_t t = threadIdx.x, lane = t & 63u, wid = t >> 6;
+    const uint32_t N = *nchunk;
+    const uint32_t cend = min(N, (blockIdx.x + 1) * per_wg);
+    for (uint32_t ci = blockIdx.x * per_wg; ci < cend;) {
+        const ChunkRef c = cr[ci];
+        const PartSeg s = ps[c.seg];
+        // this segment's chunks c.idx, c.idx + 1, .. inside the range: one stream
+        const uint32_t nst = min(cend - ci, s.nchunks - c.idx);
+        ci += nst;
+        const uint32_t B = 2u << s.logf;
+        const uint64_t lo = (uint64_t)c.idx * s.chunk;
+        const uint64_t hi = min(lo + (uint64_t)nst * s.chunk, s.size);
+        const T* src = (s.buf ? b : a) + s.begin;
+        T* dst = (s.buf ? a : b) + s.begin;
+        const uint16_t* isrc = ids + s.begin;
+        // stores through an explicit global pointer: a flat store would also
+        // count against the LDS wait counter and stall the next LDS read
+        __attribute__((address_space(1))) T* gdst = (__attribute__((address_space(1))) T*)dst;
+        const uint32_t dph = (uint32_t)((uintptr_t)dst / sizeof(T));  // line arithmetic on the address
+        const uint32_t* h = hist + s.hist_base + (uint64_t)c.idx * B;
+        for (uint32_t j = t; j < BMAX; j += P) {
+            sm.cursor[j] = j < B ? (uint32_t)bstart[s.bkt_base + j] + h[j] : 0u;
+            sm.rh[j] = 0;
+            sm.rec[j] = Rec{};  // nothing carried or waiting
+        }
+        struct Buf {
+            T x[E];
+            uint32_t b[E];
+        };
+        // the prefetch after the last round reads key hi - 1 everywhere: in bounds
+        auto load_round = [&](uint64_t base, Buf& R) {
+            const uint64_t bb = base < hi ? base : hi - 1;
+            const uint32_t cnt = (uint32_t)((hi - bb) < (uint64_t)ROUND ? (hi - bb) : (uint64_t)ROUND);
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const uint32_t i = e * P + t;
+                const uint64_t ic = bb + (i < cnt ? i : cnt - 1);
+                R.x[e] = src[ic];
+                R.b[e] = isrc[ic];
+            }
+        };
+        auto do_round = [&](uint64_t base, Buf& cur, Buf& nxt) {
+            const uint32_t cnt = (uint32_t)((hi - base) < (uint64_t)ROUND ? (hi - base) : (uint64_t)ROUND);
+            const bool last = base + ROUND >= hi;
+            load_round(base + ROUND, nxt);
+            uint32_t bk[E], lr[E];
+#pragma unroll
+            for (int e = 0; e < E; ++e) bk[e] = cur.b[e] & (BMAX - 1u);
+            if (cnt == ROUND) {
+                wave_rank_many<E>(sm.rh, sm.dummy, bk, lr);
+            } else {
+#pragma unroll
+                for (int e = 0; e < E; ++e)
+                    if (e * P + t < cnt) lr[e] = atomicAdd(&sm.rh[bk[e]], 1u);
+            }
+            __syncthreads();  // 1
+            // ---- owner of ordinary bucket jo = 2t and equality bucket je = 2t + 1
+            const uint32_t jo = 2 * t, je = jo + 1;
+            uint32_t kk;  // keys carried into this round
+            {
+                // the previous round's waiting keys of bucket jo: staged -> carry
+                const uint4 rv = *reinterpret_cast<const uint4*>(&sm.rec[jo]);
+                const uint32_t ts = rv.z >> 16, tn = (rv.w >> 16) & 0xFFu, cd = rv.w >> 24;
+                T tmp[CW];
+#pragma unroll
+                for (int q = 0; q < CW; ++q)
+                    if (q < (int)tn) tmp[q] = sm.stg[ts + q];
+#pragma unroll
+                for (int q = 0; q < CW; ++q)
+                    if (q < (int)tn) sm.carry[t * CW + cd + q] = tmp[q];
+                kk = cd + tn;
+            }
+            const uint32_t ho = sm.rh[jo], he = sm.rh[je];
+            const uint32_t curo = sm.cursor[jo], cure = sm.cursor[je];
+            const uint32_t d_o = curo - kk, eo = curo + ho;
+            // written: up to the last line boundary at or before the end (then
+            // wo >= kk, the carried keys lie in the line that starts at d_o)
+            const uint32_t nk = kk + ho, past = (dph + eo) & (LINE - 1u);
+            const uint32_t wo = last ? nk : (nk > past ? nk - past : 0u);
+            const uint32_t ao = (dph + d_o) & (LINE - 1u), ae = (dph + cure) & (LINE - 1u);
+            const uint32_t lo_ = wo ? (ao + wo + LINE - 1) / LINE : 0u;
+            const uint32_t le_ = he ? (ae + he + LINE - 1) / LINE : 0u;
+            const uint32_t v = (ho + he) | ((lo_ + le_) << 16);  // keys | jobs << 16 (each total < 65536)
+            uint32_t incl = v;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t y = __shfl_up(incl, d);
+                if (lane >= (uint32_t)d) incl += y;
+            }
+            if (lane == 63) sm.wtot[wid] = incl;
+            __syncthreads();  // 2
+            uint32_t woff = 0, tot = 0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+                const uint32_t x = sm.wtot[w];
+                woff += (w < (int)wid) ? x : 0u;
+                tot += x;
+            }
+            const uint32_t p = woff + incl - v;
+            const uint32_t rso = p & 0xFFFFu, jso = p >> 16;
+            const uint32_t rse = rso + ho, jse = jso + lo_;
+            {
+                Rec r;
+                r.d = d_o;
+                r.w = (uint16_t)wo;
+                r.rs = (uint16_t)rso;
+                r.js = (uint16_t)jso;
+                r.k = (uint8_t)kk;
+                r.a = (uint8_t)ao;
+                if (wo) {  // the staged keys past the written ones wait, from carry slot 0
+                    r.ts = (uint16_t)(rso + (wo - kk));
+                    r.tn = (uint8_t)(ho - (wo - kk));
+                    r.cd = 0;
+                } else {   // nothing written: all of them wait, after the carried ones
+                    r.ts = (uint16_t)rso;
+                    r.tn = (uint8_t)ho;
+                    r.cd = (uint8_t)kk;
+                }
+                sm.rec[jo] = r;
+                Rec q;
+                q.d = cure;
+                q.w = (uint16_t)he;
+                q.rs = (uint16_t)rse;
+                q.js = (uint16_t)jse;
+                q.k = 0;
+                q.a = (uint8_t)ae;
+                q.ts = 0;
+                q.tn = 0;
+                q.cd = 0;
+                sm.rec[je] = q;
+            }
+            // bucket of every line job: the owner writes those of a bucket with
+            // few lines; the whole wave writes those of each bucket with many
+            // (sorted input puts a round's keys in one or two buckets)
+            constexpr uint32_t FEW = 8;
+            if (lo_ <= FEW)
+                for (uint32_t i = 0; i < lo_; ++i) sm.jb[jso + i] = (uint16_t)jo;
+            if (le_ <= FEW)
+                for (uint32_t i = 0; i < le_; ++i) sm.jb[jse + i] = (uint16_t)je;
+            for (uint64_t many = __ballot(lo_ > FEW); many; many &= many - 1) {
+                const int src_lane = __ffsll((long long)many) - 1;
+                const uint32_t js = __shfl(jso, src_lane), n = __shfl(lo_, src_lane);
+                for (uint32_t i = lane; i < n; i += 64) sm.jb[js + i] = (uint16_t)(2 * (64 * wid + src_lane));
+            }
+            for (uint64_t many = __ballot(le_ > FEW); many; many &= many - 1) {
+                const int src_lane = __ffsll((long long)many) - 1;
+                const uint32_t js = __shfl(jse, src_lane), n = __shfl(le_, src_lane);
+                for (uint32_t i = lane; i < n; i += 64) sm.jb[js + i] = (uint16_t)(2 * (64 * wid + src_lane) + 1);
+            }
+            sm.cursor[jo] = eo;
+            sm.cursor[je] = cure + he;
+            sm.rh[jo] = 0;
+            sm.rh[je] = 0;
+            __syncthreads();  // 3
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const uint32_t i = e * P + t;
+                if (i < cnt) sm.stg[sm.rec[bk[e]].rs + lr[e]] = cur.x[e];
+            }
+            __syncthreads();  // 4
+            // line jobs, U per lane group at once: the LDS reads of U jobs are in
+            // flight together (each job is a chain of three dependent reads)
+            const uint32_t njobs = tot >> 16;
+            const uint32_t g = lane / LINE, l = lane % LINE;
+            constexpr int U = 4;
+            for (uint32_t q0 = wid * JPI + g; q0 < njobs; q0 += NW * JPI * U) {
+                uint32_t jq[U];
+                uint4 rv[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t q = q0 + (uint32_t)u * NW * JPI;
+                    jq[u] = q < njobs ? sm.jb[q] : 0u;
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) rv[u] = *reinterpret_cast<const uint4*>(&sm.rec[jq[u]]);
+                T y[U];
+                uint32_t dp[U];
+                bool ok[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t q = q0 + (uint32_t)u * NW * JPI;
+                    const uint32_t w = rv[u].y & 0xFFFFu, rs = rv[u].y >> 16, js = rv[u].z & 0xFFFFu;
+                    const uint32_t k = rv[u].w & 0xFFu, a = (rv[u].w >> 8) & 0xFFu;
+                    const int32_t o = (int32_t)((q - js) * LINE + l) - (int32_t)a;
+                    ok[u] = q < njobs && o >= 0 && o < (int32_t)w;
+                    const T* p = (uint32_t)o < k ? &sm.carry[(jq[u] >> 1) * CW + (uint32_t)o] : &sm.stg[rs + (uint32_t)o - k];
+                    y[u] = ok[u] ? *p : T(0);
+                    dp[u] = rv[u].x + (uint32_t)o;
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (ok[u]) gdst[dp[u]] = y[u];
+            }
+            // the next round writes rec, jb, stg and carry only after its
+            // barrier 1, which every thread reaches after these stores
+        };
+        Buf A, Bb;
+        load_round(lo, A);
+        __syncthreads();
+        for (uint64_t base = lo; base < hi;) {
+            do_round(base, A, Bb);
+            base += ROUND;
+            if (base >= hi) break;
+            do_round(base, Bb, A);
+            base += ROUND;
+        }
+        __syncthreads();  // LDS is reloaded for the next stream
+    }
+}
+
 // ---------------------------------------------------------------- line-buffered scatter
 // Keys only.  The rounds of k_scatter (classify or stored ids, LDS ranks,
 // staging grouped by bucket), but a 128-byte line of an ordinary bucket's run
@@ -1198,9 +1482,10 @@ __global__ __launch_bounds__(P) void k_scatter(const PartSeg* ps, const ChunkRef
 // key's place in its line), and in the round whose keys complete the line they
 // are stored just before those keys, so the line reaches the L2 as back-to-back
 // stores.  Measured on MI355X (2 GiB, up to 4096 sequential streams per CU):
-// whole 128-byte lines, or two halves stored back to back, stream at
-// 4.3-4.9 TB/s; runs of 16 fp64 keys off line alignment, the pattern of a
-// plain multiway scatter, at 1.6-1.9 TB/s.  Only the first line of each
+// whole 128-byte lines stream at 4.3-5.7 TB/s; runs of 16 fp64 keys off line
+// alignment, the pattern of a plain multiway scatter, at 1.6-1.9 TB/s, and
+// storing a line's waiting part at the start of the round that completes it
+// (this kernel's pattern) measured no faster than that.  Only the first line of each
 // (chunk, bucket) run (shared with the previous chunk) and its last line at
 // the chunk's end are stored partially, so the planner gives this kernel
 // large chunks.  Equality buckets (odd) are stored directly.
@@ -1437,6 +1722,18 @@ __global__ __launch_bounds__(P) void k_scatter_lb(const PartSeg* ps, const Chunk
 
 // Scatter variants: the workgroup shape, and whether the bucket of each key
 // is read from the ids the count pass stored or classified again.
+// Compute units of the current device (cached per device).
+inline uint32_t device_cu_count() {
+    static int cache[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) return 256;
+    if (dev < 64 && cache[dev] > 0) return (uint32_t)cache[dev];
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+    if (dev < 64) cache[dev] = cus;
+    return (uint32_t)cus;
+}
+
 template <class T, bool KV>
 inline void launch_scatter(int shape, bool use_ids, uint32_t grid, hipStream_t st, const PartSeg* ps,
                            const ChunkRef* cr, T* a, T* b, int32_t* va, int32_t* vb,
@@ -1447,6 +1744,14 @@ inline void launch_scatter(int shape, bool use_ids, uint32_t grid, hipStream_t s
     k_scatter<T, KV, I, PP, EE, W><<<grid, PP, 0, st>>>(ps, cr, a, b, va, vb, trees, guides, hist, bstart, nchunk, \
                                                         ids)
     if constexpr (!KV) {
+        if (shape == SCATTER_FL && use_ids) {
+            // persistent: about one workgroup per CU, each on a contiguous range of chunks
+            const uint32_t cus = device_cu_count();
+            const uint32_t per = (grid + cus - 1) / cus;
+            k_scatter_fl<T, 512, 16><<<(grid + per - 1) / per, 512, 0, st>>>(ps, cr, a, b, hist, bstart, nchunk,
+                                                                             ids, per);
+            return;
+        }
         if (shape == SCATTER_LB) {
             // 4096-key rounds beside 8-byte keys (the line buffers take 60 KiB), 8192 beside 4-byte keys
             constexpr int LE = sizeof(T) == 8 ? 8 : 16;
@@ -1457,7 +1762,7 @@ inline void launch_scatter(int shape, bool use_ids, uint32_t grid, hipStream_t s
             return;
         }
     }
-    if (shape == SCATTER_LB) shape = 0;  // key/value sorts: no line buffers beside the values
+    if (shape == SCATTER_LB || shape == SCATTER_FL) shape = 0;  // key/value sorts; FL without stored ids
     if (use_ids) {
         switch (shape) {
             case 1: S512_SCATTER(true, 256, 16, false); break;

@@ -709,7 +709,10 @@ EngineParams params_from(const s512_options* opt, bool wide) {
     if (opt->imbalance_pct) p.imbalance = (float)opt->imbalance_pct / 100.0f;
     if (opt->scatter_variant) {
         const uint32_t v = opt->scatter_variant - 1;
-        if (v >= 2 * SCATTER_SHAPES) {  // 9, 10: line-buffered, classified again / stored ids
+        if (v == 2 * SCATTER_SHAPES + 2) {  // 11: full lines, stored ids
+            p.scatter_shape = SCATTER_FL;
+            p.ids = true;
+        } else if (v >= 2 * SCATTER_SHAPES) {  // 9, 10: line-buffered, classified again / stored ids
             p.scatter_shape = SCATTER_LB;
             p.ids = v == 2 * SCATTER_SHAPES + 1;
         } else {
@@ -797,8 +800,8 @@ static s512_status check_opts(const s512_options* opt) {
         set_error("count_variant %u out of range (max %d)", opt->count_variant, COUNT_SHAPES);
         return S512_ERR_INVALID;
     }
-    if (opt && opt->scatter_variant > 2 * SCATTER_SHAPES + 2) {
-        set_error("scatter_variant %u out of range (max %d)", opt->scatter_variant, 2 * SCATTER_SHAPES + 2);
+    if (opt && opt->scatter_variant > 2 * SCATTER_SHAPES + 3) {
+        set_error("scatter_variant %u out of range (max %d)", opt->scatter_variant, 2 * SCATTER_SHAPES + 3);
         return S512_ERR_INVALID;
     }
     if (opt && opt->target_leaf && opt->target_leaf < 64) {

@@ -218,7 +218,12 @@ typedef struct s512_options {
                                  stored (2 bytes per key); 9 and 10: the line-buffered
                                  scatter (every 128-byte line of a bucket's run stored
                                  whole; keys-only sorts, key/value sorts run shape 0),
-                                 9 classifying again, 10 reading the stored ids         */
+                                 9 classifying again, 10 reading the stored ids; 11: the
+                                 full-line scatter (every store covers whole 128-byte
+                                 lines, a bucket's unfinished line waits in LDS for the
+                                 next round; persistent workgroups over contiguous chunk
+                                 ranges; stored ids; keys-only sorts, key/value sorts run
+                                 shape 0)                                               */
     uint32_t count_variant;   /* 0: default (shape 2); else 1 + count shape 0..4 = (threads,
                                  keys per thread per batch, keys classified together):
                                  256x16x8, 256x16x4, 256x8x8, 512x8x8, 256x8x4          */

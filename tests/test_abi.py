@@ -53,7 +53,7 @@ def test_host_validation_without_gpu(s512):
     opt = s512.Options()
     opt.oversample = 100000
     assert lib.s512_sort_i32_dev_ex(ctypes.c_void_p(16), 100, ctypes.byref(opt), None, None) == 1
-    for field, bad in (("scatter_variant", 11), ("count_variant", 6), ("tile_variant", 4)):
+    for field, bad in (("scatter_variant", 12), ("count_variant", 6), ("tile_variant", 4)):
         opt = s512.Options()
         setattr(opt, field, bad)
         assert lib.s512_sort_f64_dev_ex(ctypes.c_void_p(16), 100, ctypes.byref(opt), None, None) == 1

@@ -40,7 +40,7 @@ def _run(s512, dev, a, **opts):
 
 @pytest.mark.parametrize("kind", ["normal", "distinct", "sorted"])
 @pytest.mark.parametrize("dtype", [np.int32, np.float64])
-@pytest.mark.parametrize("variant", range(1, 11))
+@pytest.mark.parametrize("variant", range(1, 12))
 def test_scatter_variants(s512, orc, dev, dtype, kind, variant):
     a = _input(orc, dtype, kind)
     got, st = _run(s512, dev, a, scatter_variant=variant)
@@ -50,10 +50,10 @@ def test_scatter_variants(s512, orc, dev, dtype, kind, variant):
 
 @pytest.mark.parametrize("offset", [0, 1, 3])
 @pytest.mark.parametrize("dtype", [np.int32, np.float64])
-@pytest.mark.parametrize("variant", [9, 10])
+@pytest.mark.parametrize("variant", [9, 10, 11])
 def test_line_buffered_scatter_unaligned(s512, orc, dev, dtype, variant, offset):
-    """The line-buffered scatter finds line boundaries from the destination's
-    address: arrays that start 1 or 3 keys past a 128-byte boundary (and the
+    """The line-buffered and full-line scatters find line boundaries from the
+    destination's address: arrays that start 1 or 3 keys past a 128-byte boundary (and the
     scratch array, which is aligned) must sort the same; 9,437,187 keys give
     many chunks per level, two levels and a ragged last round."""
     import torch
@@ -69,10 +69,10 @@ def test_line_buffered_scatter_unaligned(s512, orc, dev, dtype, variant, offset)
     assert_keys_equal(to_host(t), orc.closed_sort(a.copy()), f"scatter_variant={variant} offset={offset}")
 
 
-@pytest.mark.parametrize("variant", [9, 10])
+@pytest.mark.parametrize("variant", [9, 10, 11])
 def test_line_buffered_scatter_wide_and_segmented(s512, dev, variant):
     """int64 keys (8-byte lines of 16 keys) and a segmented batch through the
-    line-buffered scatter, against numpy's sort (the ascending order is
+    line-buffered and full-line scatters, against numpy's sort (the ascending order is
     unique)."""
     import torch
     rng = np.random.default_rng(8)
@@ -134,7 +134,7 @@ def test_fp64_minmax_tiles_with_variants(s512, orc, dev):
 def test_variant_out_of_range(s512, dev):
     import torch
     t = torch.zeros(10, dtype=torch.float64, device=dev)
-    for field, bad in (("scatter_variant", 11), ("count_variant", 6), ("tile_variant", 4)):
+    for field, bad in (("scatter_variant", 12), ("count_variant", 6), ("tile_variant", 4)):
         o = s512.Options()
         setattr(o, field, bad)
         with pytest.raises(s512.S512Error):
