@@ -90,7 +90,7 @@ struct PartSeg {
     uint64_t guide_base; // offset of this segment's guide table ([8F] cells)
     uint64_t kmin;       // guide table origin (encoded key), set on the device
     uint32_t shift;      // guide cell width 2^shift (encoded units), set on the device
-    uint32_t mode;       // 0: sampled splitters, 1: learned model
+    uint32_t mode;       // 0: sampled splitters, 2: the learned model's CDF table
     uint32_t lshift;     // learned: bucket = (fine bucket >> lshift) - lbase
     uint32_t lbase;
     uint32_t flags;      // classifier flags (GUIDE_SUBTABLES, GUIDE_EQBUCKETS), set on the device

@@ -9,14 +9,13 @@
 
 namespace s512 {
 
-// State of the learned partitioner across the level loop: the model places
-// the splitters of the root level (2^lf1 buckets) and of its buckets.
+// State of the learned partitioner across the level loop: the model's CDF
+// tables place the keys of the root level (2^lf1 buckets) and of its buckets.
 struct LevelState {
     bool use_learned = false;
     uint32_t lf1 = 0;
-    LearnedModelDev l1{nullptr, nullptr};  // knots placing the first level's splitters
-    LearnedModelDev l2{nullptr, nullptr};  // knots placing the second level's splitters
-    hipEvent_t trained = nullptr;          // l2 is ready once this event completes
+    LearnedModelDev tabs{nullptr, nullptr};  // the root's CDF table, the root buckets' local tables
+    hipEvent_t trained = nullptr;            // tabs.tab2 is ready once this event completes
     const float* loss_host = nullptr;      // training loss (pinned), valid after `trained`
     float max_loss = 0.0f;
     bool pending = false;                  // the second level still has to wait for `trained`

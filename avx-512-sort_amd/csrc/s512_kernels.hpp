@@ -162,9 +162,17 @@ __device__ __forceinline__ uint32_t lower_bound_lds(const K* srt, uint32_t n, K 
 
 // Load a segment's classifier into LDS: splitters + guide table.
 constexpr int LDS_TABLE_WORDS = GMAX;
+static_assert(CT_WORDS <= LDS_TABLE_WORDS, "the learned CDF table fits the guide's LDS");
 template <class K, int NT>
 __device__ __forceinline__ Guide load_classifier(const PartSeg& s, const K* trees, const uint32_t* guides, K* lsrt,
                                                  uint32_t* lguide) {
+    if (s.mode == 2) {
+        // the learned model's CDF table (s512_learned.hpp), no splitters: the
+        // root's tree table, or below the root the bucket's local table
+        const uint32_t words = s.lshift ? LT_WORDS : CT_WORDS;
+        for (uint32_t j = threadIdx.x; j < words; j += NT) lguide[j] = guides[s.guide_base + j];
+        return Guide{0, 0, 0, 0, 0, 0};
+    }
     const uint32_t F = 1u << s.logf, G = guide_words(F);
     for (uint32_t j = threadIdx.x; j < F; j += NT) lsrt[j] = trees[s.tree_base + j];
     for (uint32_t j = threadIdx.x; j < G; j += NT) lguide[j] = guides[s.guide_base + j];
@@ -242,6 +250,29 @@ __device__ __forceinline__ void classify_batch(const K* srt, const uint32_t* gd,
     }
 }
 
+
+// Buckets of keys of segment s: splitters and guide (modes 0, 1), or the
+// learned CDF table in O(1) per key (mode 2: ordinary buckets only).
+template <class K, int NB>
+__device__ __forceinline__ void classify_keys(const PartSeg& s, const K* srt, const uint32_t* gd, const Guide& g,
+                                              const K (&x)[NB], uint32_t (&bk)[NB]) {
+    if (s.mode == 2) {
+        if (s.lshift) {
+#pragma unroll
+            for (int u = 0; u < NB; ++u) bk[u] = local_bucket(gd, s.logf, x[u]);
+        } else {
+            table_buckets<K, NB>(gd, s.logf, x, bk);
+        }
+    } else {
+        classify_batch<K, NB>(srt, gd, g, x, bk);
+    }
+}
+template <class K>
+__device__ __forceinline__ uint32_t classify_key(const PartSeg& s, const K* srt, const uint32_t* gd, const Guide& g,
+                                                 K x) {
+    if (s.mode == 2) return s.lshift ? local_bucket(gd, s.logf, x) : table_bucket(gd, s.logf, x);
+    return classify(srt, gd, g, x);
+}
 
 // ---------------------------------------------------------------- tile kernels
 // How keys of type T enter and leave a network ordered by policy Pol.
@@ -517,51 +548,13 @@ __global__ __launch_bounds__(256) void k_sample(const PartSeg* ps, const uint32_
     using Tr = KeyTraits<T>;
     for (uint32_t w = blockIdx.x; w < *nseg; w += gridDim.x) {
         const PartSeg s = ps[w];
-        if (s.mode) continue;  // learned segment: k_model_splitters writes its splitters
+        if (s.mode) continue;  // learned segment: classified by the model's CDF table
         const T* src = (s.buf ? b : a) + s.begin;
         typename Tr::K* out = samples + s.sample_base;
         for (uint32_t i = threadIdx.x; i < s.nsample; i += blockDim.x) {
             const uint64_t h = mix64(seed ^ (s.begin * 0x632BE59BD9B4E019ull), i);
             const uint64_t pos = __umul64hi(h, s.size);  // uniform in [0, size)
             out[i] = Tr::enc(src[pos]);
-        }
-    }
-}
-
-// Learned segments (mode 1): the splitters are quantiles of the learned CDF
-// model (s512_learned.hpp) instead of a sorted random sample.  Quantile range
-// of a segment: the learned root (lshift = 0) covers [0, 1); bucket j
-// (= lbase) of a 2^lshift-way learned root covers [j, j + 1) / 2^lshift.
-// Written as a sorted pseudo-sample of F keys -- the F - 1 quantiles, then the
-// range's upper key -- which k_build_tree turns into splitters exactly as it
-// does a sample of F keys.
-//
-// `root` is the knot table that placed the root level; `model` places this
-// level.  When they differ (the root from the sample's own CDF, the second
-// level from the trained network over the same knot keys), bucket j's range
-// is mapped into the model's CDF at the root's boundary keys, so the second
-// level subdivides exactly the keys bucket j holds.
-template <class K>
-__global__ __launch_bounds__(256) void k_model_splitters(const PartSeg* ps, const uint32_t* nseg,
-                                                         LearnedModelDev model, LearnedModelDev root, K* samples) {
-    for (uint32_t w = blockIdx.x; w < *nseg; w += gridDim.x) {
-        const PartSeg s = ps[w];
-        if (!s.mode) continue;
-        const uint32_t F = 1u << s.logf;
-        const double den = (double)(1ull << s.lshift);
-        double q0 = (double)s.lbase / den, q1 = (double)(s.lbase + 1) / den;
-        if (s.lshift && root.cdf != model.cdf) {
-            uint32_t lo;
-            double f;
-            model_position(root.cdf, q0, &lo, &f);
-            q0 = s.lbase ? model_cdf_at(model.cdf, lo, f) : 0.0;
-            model_position(root.cdf, q1, &lo, &f);
-            q1 = s.lbase + 1 < (1u << s.lshift) ? model_cdf_at(model.cdf, lo, f) : 1.0;
-        }
-        for (uint32_t t = threadIdx.x; t < F; t += blockDim.x) {
-            const double q = t + 1 < F ? q0 + (q1 - q0) * (double)(t + 1) / (double)F : q1;
-            samples[s.sample_base + t] =
-                (K)(model.smp ? model_quantile_rank(model, q) : model_quantile(model.key, model.cdf, q));
         }
     }
 }
@@ -816,7 +809,7 @@ __global__ __launch_bounds__(CNT_P) void k_count(const PartSeg* ps, const ChunkR
                 uint32_t kb[NB];
 #pragma unroll
                 for (int u = 0; u < NB; ++u) xb[u] = x[u0 + u];
-                classify_batch<K, NB>(lsrt, lguide, g, xb, kb);
+                classify_keys<K, NB>(s, lsrt, lguide, g, xb, kb);
 #pragma unroll
                 for (int u = 0; u < NB; ++u) bk[u0 + u] = kb[u];
             }
@@ -834,7 +827,7 @@ __global__ __launch_bounds__(CNT_P) void k_count(const PartSeg* ps, const ChunkR
             for (int u = 0; u < CNT_U; ++u) {
                 const uint64_t i = base + (uint64_t)u * CNT_P + t;
                 if (i < hi) {
-                    const uint32_t bk = classify(lsrt, lguide, g, x[u]);
+                    const uint32_t bk = classify_key(s, lsrt, lguide, g, x[u]);
                     if (special) odd |= nan_or_negzero(x[u]);
                     atomicAdd(&lh[bk], 1u);
                     if (oid) oid[i] = (uint16_t)bk;
@@ -1077,13 +1070,13 @@ __global__ __launch_bounds__(P) void k_scatter(const PartSeg* ps, const ChunkRef
                 uint32_t kb[CLS_NB];
 #pragma unroll
                 for (int u = 0; u < CLS_NB; ++u) kx[u] = Tr::enc(cur.x[e0 + u]);
-                classify_batch<K, CLS_NB>(sm.lsrt, sm.lguide, g, kx, kb);
+                classify_keys<K, CLS_NB>(s, sm.lsrt, sm.lguide, g, kx, kb);
 #pragma unroll
                 for (int u = 0; u < CLS_NB; ++u) bk[e0 + u] = kb[u];
             }
         } else {
 #pragma unroll
-            for (int e = 0; e < E; ++e) bk[e] = classify(sm.lsrt, sm.lguide, g, Tr::enc(cur.x[e]));
+            for (int e = 0; e < E; ++e) bk[e] = classify_key(s, sm.lsrt, sm.lguide, g, Tr::enc(cur.x[e]));
         }
         if (cnt == ROUND) {
             wave_rank_many<E>(sm.rh, sm.dummy, bk, lr);
@@ -1609,13 +1602,13 @@ __global__ __launch_bounds__(P) void k_scatter_lb(const PartSeg* ps, const Chunk
                 uint32_t kb[CLS_NB];
 #pragma unroll
                 for (int u = 0; u < CLS_NB; ++u) kx[u] = Tr::enc(cur.x[e0 + u]);
-                classify_batch<K, CLS_NB>(sm.lsrt, sm.lguide, g, kx, kb);
+                classify_keys<K, CLS_NB>(s, sm.lsrt, sm.lguide, g, kx, kb);
 #pragma unroll
                 for (int u = 0; u < CLS_NB; ++u) bk[e0 + u] = kb[u];
             }
         } else {
 #pragma unroll
-            for (int e = 0; e < E; ++e) bk[e] = classify(sm.lsrt, sm.lguide, g, Tr::enc(cur.x[e]));
+            for (int e = 0; e < E; ++e) bk[e] = classify_key(s, sm.lsrt, sm.lguide, g, Tr::enc(cur.x[e]));
         }
         if (cnt == ROUND) {
             wave_rank_many<E>(sm.rh, sm.dummy, bk, lr);
@@ -1789,7 +1782,8 @@ inline void launch_scatter(int shape, bool use_ids, uint32_t grid, hipStream_t s
 struct PlanLists {
     TileSeg* tile[8];         // per tile class
     uint32_t* tile_count;     // [8]
-    uint64_t* tile_keys;      // [8] keys per class; [8] keys copied back
+    uint64_t* tile_keys;      // [8] keys per class; [8] keys copied back; [10] largest
+                              // bucket of the learned model's second level
     uint32_t tile_cap[8];     // class capacities (keys)
     uint32_t nclass;
     TileSeg* copy;            // pieces to copy scratch -> caller's array
@@ -1807,15 +1801,13 @@ struct PlanLists {
                                                                      PlanLists pl) {
     constexpr int NL = 10;  // lists: tile classes 0..7, copy = 8, next = 9
     __shared__ uint32_t lcnt[NL], lbase[NL];
-    __shared__ unsigned long long lkeys[NL];
+    __shared__ unsigned long long lkeys[NL + 2];  // + the learned second level's largest bucket at 11
     for (uint32_t w = blockIdx.x; w < *nseg; w += gridDim.x) {
     const PartSeg s = ps[w];
     const uint32_t B = 2u << s.logf;
     const uint32_t b = blockIdx.y * blockDim.x + threadIdx.x;
-    if (threadIdx.x < NL) {
-        lcnt[threadIdx.x] = 0;
-        lkeys[threadIdx.x] = 0;
-    }
+    if (threadIdx.x < NL + 2) lkeys[threadIdx.x] = 0;
+    if (threadIdx.x < NL) lcnt[threadIdx.x] = 0;
     __syncthreads();
     int list = -1;
     uint32_t rank = 0, np = 1;
@@ -1840,8 +1832,12 @@ struct PlanLists {
             rank = atomicAdd(&lcnt[list], np);
             atomicAdd(&lkeys[list], (unsigned long long)sz);
         }
+        // largest bucket the learned model placed below its root (statistics):
+        // the workgroup's largest, then one global atomic
+        if (s.mode == 2 && s.lshift) atomicMax(&lkeys[11], (unsigned long long)sz);
     }
     __syncthreads();
+    if (threadIdx.x == 0 && lkeys[11]) atomicMax((unsigned long long*)&pl.tile_keys[10], lkeys[11]);
     if (threadIdx.x < NL && lcnt[threadIdx.x]) {
         const uint32_t l = threadIdx.x;
         uint32_t* cnt = l < 8 ? &pl.tile_count[l] : (l == 8 ? pl.copy_count : pl.next_count);

@@ -1,22 +1,31 @@
-// s512_learned.hip -- GPU training of the learned partitioner's CDF model and
-// its knots (see s512_learned.hpp for the model).  No reference counterpart.
+// s512_learned.hip -- the learned partitioner's CDF model: its table and the
+// GPU training of its network (see s512_learned.hpp).  No reference
+// counterpart.
 //
-//   k_learn_sample   gather ~1% of the keys (hashed positions), encoded
-//   (plain sort)     the sample is sorted by this library's own sort engine
-//   k_learn_prep     features u and targets (rank+0.5)/ns of M evenly spaced
-//                    sample points; parameters initialised so that the net is
-//                    the piecewise-linear interpolation through H+1 sample
-//                    quantiles
+//   k_learn_sample   gather ~1% of the keys (runs of 8 at hashed positions), encoded
+//   k_tab_range      smallest and largest sampled key
+//   k_tab_hist       sampled keys per top cell (LDS histograms), with each
+//                    cell's smallest and largest key (repeated values)
+//   k_tab_top        cumulative counts: the CDF at every top-cell boundary;
+//                    the heaviest cells become nodes (split in CT_FAN)
+//   k_tab_nodehist   sampled keys per sub-cell of the newest nodes    } once
+//   k_tab_nodes      their CDF values; the heaviest sub-cells become   } per
+//                    nodes of the next level                            } level
+//   k_tab_train_points  the network's training points (features at 256
+//                    evenly spaced CDF values of the table) and its initial
+//                    parameters (the piecewise-linear interpolation through
+//                    33 of them)
 //   k_mlp_train      every Adam step in one workgroup (one point per thread),
 //                    on a stream of its own beside the first partition level:
 //                    forward + backward, the gradient summed in a fixed order
 //                    (deterministic), the update; the final full-batch loss
-//   k_knots          knot keys at equal sample mass, the network's CDF there
-//                    kept within the sample's band
-//   k_knot_scan      running maximum: the knot CDF values made non-decreasing
+//   k_tab_mlp        the second level's table: the network's CDF at the same
+//                    boundaries, kept within the sample's band, non-decreasing
+//   k_tab_local      each root bucket's local table (the network's CDF over the
+//                    bucket's key range, renormalised) for the second level
+// No step sorts the sample: the table is built from histograms.
 // k_mlp_step is the same gradient spread over many workgroups, for the
 // gradient check of s512_mlp_loss_grad_f32 on more points than one workgroup.
-#include "s512_learned.hpp"
 #include "s512_kernels.hpp"
 #include "s512_runtime.hpp"
 #include "s512_engine.hpp"
@@ -26,44 +35,20 @@
 
 namespace s512 {
 
+// Runs of `run` consecutive keys (up to one 64-byte line of 8-byte keys) at
+// hashed positions: the gather reads whole lines instead of one key per line.
+// The host keeps at least 512 runs per first-level bucket (consecutive keys of
+// sorted input are one sample's worth of information, not `run`).
+constexpr uint32_t LEARN_RUN_MAX = 8;
 template <class T>
-__global__ __launch_bounds__(256) void k_learn_sample(const T* a, uint64_t n, uint32_t ns, uint64_t seed,
-                                                      typename KeyTraits<T>::K* out) {
+__global__ __launch_bounds__(256) void k_learn_sample(const T* a, uint64_t n, uint32_t ns, uint32_t run,
+                                                      uint64_t seed, typename KeyTraits<T>::K* out) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ns) return;
-    const uint64_t pos = __umul64hi(mix64(seed, i), n);
-    out[i] = KeyTraits<T>::enc(a[pos]);
-}
-
-// One block.  s: sorted encoded sample of ns keys.  Writes u[M], t[M],
-// params[MLP_NP] (initial), and geo = {kmin, kmax} of the sample.
-template <class K>
-__global__ __launch_bounds__(256) void k_learn_prep(const K* s, uint32_t ns, uint32_t M, float* u, float* t,
-                                                    float* params, uint64_t* geo) {
-    const uint64_t kmin = (uint64_t)s[0], kmax = (uint64_t)s[ns - 1];
-    const float inv = kmax > kmin ? 1.0f / (float)(kmax - kmin) : 0.0f;
-    for (uint32_t i = threadIdx.x; i < M; i += blockDim.x) {
-        const uint32_t r = (uint32_t)((((uint64_t)2 * i + 1) * ns) / (2ull * M));
-        u[i] = learned_feature(s[r], kmin, inv);
-        t[i] = ((float)r + 0.5f) / (float)ns;
-    }
-    if (threadIdx.x < MLP_H) {
-        const int j = threadIdx.x;
-        const uint32_t r0 = (uint32_t)(((uint64_t)j * (ns - 1)) / MLP_H);
-        const uint32_t r1 = (uint32_t)(((uint64_t)(j + 1) * (ns - 1)) / MLP_H);
-        const float k0 = learned_feature(s[r0], kmin, inv), k1 = learned_feature(s[r1], kmin, inv);
-        const float w = 1.0f / fmaxf(k1 - k0, 1e-6f);
-        params[j] = sqrtf(w);                        // p1
-        params[MLP_OFF_B1 + j] = -k0 * w;            // b1
-        params[MLP_OFF_B2 + j] = 0.0f;               // b2
-        params[MLP_OFF_P3 + j] = sqrtf(1.0f / MLP_H);
-        for (int i = 0; i < MLP_H; ++i) params[MLP_OFF_P2 + j * MLP_H + i] = (i == j) ? 1.0f : 0.02f;
-    }
-    if (threadIdx.x == 0) {
-        params[MLP_OFF_B3] = 0.0f;
-        geo[0] = kmin;
-        geo[1] = kmax;
-    }
+    const uint64_t starts = n > run ? n - run + 1 : 1;
+    const uint64_t start = __umul64hi(mix64(seed, i / run), starts);
+    const uint64_t pos = start + i % run;
+    out[i] = KeyTraits<T>::enc(a[pos < n ? pos : n - 1]);
 }
 
 struct TrainBufs {
@@ -309,20 +294,6 @@ __global__ __launch_bounds__(TRAIN_P) void k_mlp_train(float* params, const floa
     if (tid == 0) *loss_out = g[MLP_NP] / (float)M;
 }
 
-// Point-mass check on the sorted sample: flag = 1 when some fp32 feature value
-// is shared by more than `run` consecutive sample keys.  The network sees a
-// key only through its feature and its CDF is continuous in it, so it cannot
-// place a bucket boundary inside such a value's keys.
-template <class K>
-__global__ __launch_bounds__(256) void k_point_mass(const K* smp, uint32_t ns, uint32_t run, const uint64_t* geo,
-                                                    uint32_t* flag) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i + run >= ns) return;
-    const uint64_t kmin = geo[0], kmax = geo[1];
-    const float inv = kmax > kmin ? 1.0f / (float)(kmax - kmin) : 0.0f;
-    if (learned_feature(smp[i], kmin, inv) == learned_feature(smp[i + run], kmin, inv)) flag[0] = 1u;
-}
-
 __device__ __forceinline__ float mlp_eval(const float* p, float x) {
     constexpr int H = MLP_H;
     float h1[H];
@@ -339,64 +310,578 @@ __device__ __forceinline__ float mlp_eval(const float* p, float x) {
     return y;
 }
 
-// Knot k: the sample key of rank r_k = k (ns - 1) / LEARN_KNOTS and the
-// network's CDF there, kept within an eighth of the sample mass of the two
-// adjacent pieces (a quarter of a knot's mass) around the empirical value
-// (r_k + 0.5) / ns.  Neighbouring bands cannot overlap, so the knots stay
-// strictly increasing: a flat stretch would place no splitter over a whole
-// knot's keys.  params = nullptr: the empirical value itself.  key = nullptr:
-// keys already written.
-template <class K>
-__global__ __launch_bounds__(256) void k_knots(const K* smp, uint32_t ns, const float* params, const uint64_t* geo,
-                                               uint64_t* key, float* cdf_raw) {
-    __shared__ float p[MLP_NP];
-    if (params)
-        for (int i = threadIdx.x; i < MLP_NP; i += blockDim.x) p[i] = params[i];
-    __syncthreads();
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k > LEARN_KNOTS) return;
-    const uint64_t kmin = geo[0], kmax = geo[1];
-    const float inv = kmax > kmin ? 1.0f / (float)(kmax - kmin) : 0.0f;
-    auto rank = [&](uint32_t j) { return (uint32_t)(((uint64_t)j * (ns - 1)) / LEARN_KNOTS); };
-    const uint32_t r = rank(k);
-    const uint32_t rl = k > 0 ? rank(k - 1) : 0u, rh = k < LEARN_KNOTS ? rank(k + 1) : ns - 1;
-    const float emp = ((float)r + 0.5f) / (float)ns;
-    const float band = 0.125f * (float)(rh - rl) / (float)ns;
-    const float y = params ? mlp_eval(p, learned_feature(smp[r], kmin, inv)) : emp;
-    if (key) key[k] = (uint64_t)smp[r];
-    cdf_raw[k] = fminf(fmaxf(fminf(fmaxf(y, emp - band), emp + band), 0.0f), 1.0f);
+// ---------------------------------------------------------------- CDF table
+struct TabScratch {
+    uint64_t* geo;      // [2] smallest, largest sampled encoded key
+    uint32_t* cnt;      // [CT_TOP] sampled keys per top cell
+    uint64_t* cmin;     // [CT_TOP] smallest sampled key of the cell
+    uint64_t* cmax;     // [CT_TOP] largest
+    uint32_t* ncnt;     // [CT_NODES][CT_FAN] sampled keys per sub-cell of a node (row k - 1)
+    uint64_t* nmin;     // [CT_NODES][CT_FAN]
+    uint64_t* nmax;     // [CT_NODES][CT_FAN]
+    uint32_t* nparent;  // [CT_NODES + 1] word of the entry node k splits
+    uint64_t* noff;     // [CT_NODES + 1] offset of node k's range from the origin
+    uint32_t* nw;       // [CT_NODES + 1] log2 of node k's width
+    uint32_t* ctl;      // [4]: nodes so far, first node of the last round, sampled keys on repeated values
+};
+
+// Top cells of width 2^shift, from an origin at a multiple of 2^shift at or
+// below kmin, cover [kmin, kmax]: (kmax - origin) >> shift < CT_TOP.  Cells on
+// multiples of 2^shift never straddle a binade of fp64 keys while shift <= 52
+// (a binade is an aligned range of 2^52 encoded keys), and nodes split them
+// on multiples of smaller powers of two, so the density of keys inside a cell
+// is that of at most one binade, where linear interpolation is exact for
+// locally uniform keys.
+struct TabGeo {
+    uint64_t origin;
+    uint32_t shift;
+};
+__host__ __device__ inline TabGeo tab_geometry(uint64_t kmin, uint64_t kmax) {
+    const uint64_t span = kmax - kmin;
+    uint32_t bits = 0;
+    while (bits < 64 && (span >> bits) != 0) ++bits;
+    uint32_t sh = bits > CT_TOP_LOG ? bits - CT_TOP_LOG : 0u;
+    for (;;) {
+        const uint64_t org = sh >= 64 ? 0ull : kmin & ~((1ull << sh) - 1ull);
+        if (sh >= 63 || ((kmax - org) >> sh) < CT_TOP) return TabGeo{org, sh};
+        ++sh;
+    }
 }
 
-// One workgroup of 1024 threads: cdf = running maximum of cdf_raw.
-__global__ __launch_bounds__(1024) void k_knot_scan(const float* cdf_raw, float* cdf) {
-    constexpr int PER = (LEARN_KNOTS + 1 + 1023) / 1024;
-    __shared__ float wmax[16];
-    const uint32_t tid = threadIdx.x;
-    float v[PER];
-    float run = 0.0f;
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-        const uint32_t k = tid * PER + q;
-        run = fmaxf(run, k <= LEARN_KNOTS ? cdf_raw[k] : 0.0f);
-        v[q] = run;
+// Fixed-point CDF (CT_CDF_BITS bits) of v sampled keys out of ns.
+__device__ __forceinline__ uint32_t cdf_fix(uint64_t v, uint64_t ns) {
+    return v >= ns ? CT_CDF_MASK : (uint32_t)((v << CT_CDF_BITS) / ns);
+}
+
+__global__ __launch_bounds__(256) void k_tab_init(TabScratch ts) {
+    for (uint32_t c = threadIdx.x; c < CT_TOP; c += blockDim.x) {
+        ts.cnt[c] = 0;
+        ts.cmin[c] = ~0ull;
+        ts.cmax[c] = 0;
     }
-    // inclusive max-scan of the per-thread maxima
-    float m = run;
-    const uint32_t lane = tid & 63u, wid = tid >> 6;
+    for (uint32_t j = threadIdx.x; j < CT_NODES * CT_FAN; j += blockDim.x) {
+        ts.ncnt[j] = 0;
+        ts.nmin[j] = ~0ull;
+        ts.nmax[j] = 0;
+    }
+    if (threadIdx.x == 0) {
+        ts.geo[0] = ~0ull;
+        ts.geo[1] = 0;
+        for (int i = 0; i < 4; ++i) ts.ctl[i] = 0;
+    }
+}
+
+template <class K>
+__global__ __launch_bounds__(256) void k_tab_range(const K* smp, uint32_t ns, uint64_t* geo) {
+    uint64_t lo = ~0ull, hi = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ns; i += gridDim.x * blockDim.x) {
+        const uint64_t x = (uint64_t)smp[i];
+        lo = x < lo ? x : lo;
+        hi = x > hi ? x : hi;
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        const uint64_t l2 = __shfl_xor(lo, d), h2 = __shfl_xor(hi, d);
+        lo = l2 < lo ? l2 : lo;
+        hi = h2 > hi ? h2 : hi;
+    }
+    __shared__ uint64_t wl[4], wh[4];  // then one atomic per block: two hot words
+    if ((threadIdx.x & 63u) == 0) {
+        wl[threadIdx.x >> 6] = lo;
+        wh[threadIdx.x >> 6] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; ++w) {
+            lo = wl[w] < lo ? wl[w] : lo;
+            hi = wh[w] > hi ? wh[w] : hi;
+        }
+        atomicMin((unsigned long long*)&geo[0], (unsigned long long)lo);
+        atomicMax((unsigned long long*)&geo[1], (unsigned long long)hi);
+    }
+}
+
+template <class K>
+__global__ __launch_bounds__(256) void k_tab_hist(const K* smp, uint32_t ns, TabScratch ts) {
+    __shared__ uint32_t lc[CT_TOP];
+    __shared__ unsigned long long lmin[CT_TOP], lmax[CT_TOP];
+    for (uint32_t c = threadIdx.x; c < CT_TOP; c += blockDim.x) {
+        lc[c] = 0;
+        lmin[c] = ~0ull;
+        lmax[c] = 0;
+    }
+    __syncthreads();
+    const TabGeo gm = tab_geometry(ts.geo[0], ts.geo[1]);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ns; i += gridDim.x * blockDim.x) {
+        const uint64_t x = (uint64_t)smp[i];
+        const uint64_t c64 = (x - gm.origin) >> gm.shift;
+        const uint32_t c = c64 < CT_TOP ? (uint32_t)c64 : CT_TOP - 1;
+        atomicAdd(&lc[c], 1u);
+        atomicMin(&lmin[c], (unsigned long long)x);
+        atomicMax(&lmax[c], (unsigned long long)x);
+    }
+    __syncthreads();
+    for (uint32_t c = threadIdx.x; c < CT_TOP; c += blockDim.x) {
+        if (!lc[c]) continue;
+        atomicAdd(&ts.cnt[c], lc[c]);
+        atomicMin((unsigned long long*)&ts.cmin[c], lmin[c]);
+        atomicMax((unsigned long long*)&ts.cmax[c], lmax[c]);
+    }
+}
+
+// Exclusive prefix of one value per thread over a workgroup of 1024 threads
+// (16 waves); *total = the sum.  Ends with a barrier.
+__device__ __forceinline__ uint32_t block_excl_1024(uint32_t v, uint32_t* wsum, uint32_t* total) {
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    uint32_t incl = v;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
-        const float o = __shfl_up(m, d);
-        if (lane >= (uint32_t)d) m = fmaxf(m, o);
+        const uint32_t y = __shfl_up(incl, d);
+        if (lane >= (uint32_t)d) incl += y;
     }
-    if (lane == 63) wmax[wid] = m;
+    if (lane == 63) wsum[wid] = incl;
     __syncthreads();
-    float before = 0.0f;
-    for (uint32_t w = 0; w < wid; ++w) before = fmaxf(before, wmax[w]);
-    const float excl = fmaxf(before, __shfl_up(m, 1) * (lane ? 1.0f : 0.0f));
+    uint32_t before = 0, all = 0;
+    for (uint32_t w = 0; w < 16; ++w) {
+        before += w < wid ? wsum[w] : 0u;
+        all += wsum[w];
+    }
+    *total = all;
+    __syncthreads();
+    return before + incl - v;
+}
+
+// Heaviest-first choice among the candidate cells of a workgroup of 1024
+// threads (N candidate counts per thread, 0 = not a candidate): the smallest
+// threshold >= lo above which at most `budget` candidates lie.  Every thread
+// returns it.
+template <int N>
+__device__ uint32_t heaviest_threshold(const uint32_t (&cand)[N], uint32_t lo, uint32_t budget, uint32_t* wsum) {
+    uint32_t mx = 0;
+#pragma unroll
+    for (int q = 0; q < N; ++q) mx = cand[q] > mx ? cand[q] : mx;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        const uint32_t o = __shfl_xor(mx, d);
+        mx = o > mx ? o : mx;
+    }
+    if ((threadIdx.x & 63u) == 0) wsum[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    uint32_t hi = 0;
+    for (int w = 0; w < 16; ++w) hi = wsum[w] > hi ? wsum[w] : hi;
+    __syncthreads();
+    if (hi < lo) hi = lo;  // nothing above lo: lo itself
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        uint32_t mine = 0, above;
+#pragma unroll
+        for (int q = 0; q < N; ++q) mine += cand[q] > mid;
+        block_excl_1024(mine, wsum, &above);
+        if (above <= budget) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
+// One workgroup of CT_TOP threads (one top cell each): the CDF at the top
+// boundaries, the header, and the first level of nodes: the heaviest cells
+// over twice the mean mass, at most CT_NODES, whose sampled keys are not all
+// one value.  Sampled keys of an unsplit cell whose keys are all one value
+// count as keys on repeated values.
+__global__ __launch_bounds__(CT_TOP) void k_tab_top(TabScratch ts, uint32_t ns, uint32_t* tab) {
+    static_assert(CT_TOP == 1024, "one cell per thread of a 1024-thread workgroup");
+    __shared__ uint32_t wsum[16];
+    const uint32_t c = threadIdx.x;
+    const uint32_t a0 = ts.cnt[c];
+    const TabGeo gm = tab_geometry(ts.geo[0], ts.geo[1]);
+    uint32_t tot;
+    const uint32_t p = block_excl_1024(a0, wsum, &tot);
+    const bool one_value = a0 >= 2 && ts.cmin[c] == ts.cmax[c];
+    const uint32_t cand[1] = {(!one_value && gm.shift >= CT_FAN_LOG) ? a0 : 0u};
+    const uint32_t thr = heaviest_threshold<1>(cand, max(2u * (ns / CT_TOP), 1u), CT_NODES, wsum);
+    const uint32_t pick = cand[0] > thr;
+    uint32_t npick;
+    const uint32_t r = block_excl_1024(pick, wsum, &npick);
+    const uint32_t id = pick ? r + 1 : 0u;
+    tab[CT_OFF_TOP + c] = (id << CT_CDF_BITS) | cdf_fix(p, ns);
+    if (c == CT_TOP - 1) tab[CT_OFF_TOP + CT_TOP] = CT_CDF_MASK;
+    if (!pick && one_value) atomicAdd(&ts.ctl[2], a0);
+    if (pick) {
+        ts.nparent[id] = CT_OFF_TOP + c;
+        ts.noff[id] = (uint64_t)c << gm.shift;
+        ts.nw[id] = gm.shift;
+    }
+    // the new nodes' entries start empty (no sub-nodes) until their round
+    for (uint32_t j = c; j < npick * (CT_FAN + 1); j += blockDim.x) tab[CT_OFF_NODE + j] = 0u;
+    if (c == 0) {
+        tab[0] = (uint32_t)gm.origin;
+        tab[1] = (uint32_t)(gm.origin >> 32);
+        tab[2] = gm.shift;
+        tab[3] = npick;
+        tab[4] = npick;  // last node of each level: words 4 .. 4 + CT_DEPTH - 1
+        for (int w = 5; w < 8; ++w) tab[w] = npick;
+        ts.ctl[0] = npick;
+        ts.ctl[1] = 1;
+    }
+}
+
+// Deepest table entry that holds offset d from the origin: its word index,
+// the node it belongs to (0: the top), and the bits left below it.
+struct TabWalk {
+    uint32_t at, node, w;
+    uint64_t rem;
+};
+__device__ __forceinline__ TabWalk tab_walk(const uint32_t* tab, uint64_t d, uint32_t shift) {
+    const uint64_t c64 = d >> shift;
+    const uint32_t c = c64 < CT_TOP ? (uint32_t)c64 : CT_TOP - 1;
+    TabWalk r{CT_OFF_TOP + c, 0u, shift, c64 < CT_TOP ? d - (c64 << shift) : (1ull << shift) - 1ull};
+    for (uint32_t lv = 0; lv < CT_DEPTH; ++lv) {
+        const uint32_t child = tab[r.at] >> CT_CDF_BITS;
+        if (!child) break;
+        const uint32_t i = (uint32_t)(r.rem >> (r.w - CT_FAN_LOG));
+        r.w -= CT_FAN_LOG;
+        r.rem &= (1ull << r.w) - 1ull;
+        r.node = child;
+        r.at = ct_node(child) + i;
+    }
+    return r;
+}
+
+// Sampled keys per sub-cell of the nodes made in the last round (their
+// entries are empty, so the walk ends in them).
+template <class K>
+__global__ __launch_bounds__(256) void k_tab_nodehist(const K* smp, uint32_t ns, TabScratch ts, const uint32_t* tab) {
+    __shared__ uint32_t lc[CT_NODES * CT_FAN];
+    __shared__ unsigned long long lmin[CT_NODES * CT_FAN], lmax[CT_NODES * CT_FAN];
+    const uint32_t first = ts.ctl[1], last = ts.ctl[0];
+    if (first > last) return;  // no new nodes (uniform: the whole grid returns)
+    const uint32_t rows = (last - first + 1) * CT_FAN;
+    for (uint32_t j = threadIdx.x; j < rows; j += blockDim.x) {
+        lc[j] = 0;
+        lmin[j] = ~0ull;
+        lmax[j] = 0;
+    }
+    __syncthreads();
+    const uint64_t origin = (uint64_t)tab[0] | ((uint64_t)tab[1] << 32);
+    const uint32_t shift = tab[2];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ns; i += gridDim.x * blockDim.x) {
+        const uint64_t x = (uint64_t)smp[i];
+        const TabWalk r = tab_walk(tab, x - origin, shift);
+        if (r.node < first) continue;
+        const uint32_t j = (r.node - first) * CT_FAN + (r.at - ct_node(r.node));
+        atomicAdd(&lc[j], 1u);
+        atomicMin(&lmin[j], (unsigned long long)x);
+        atomicMax(&lmax[j], (unsigned long long)x);
+    }
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < rows; j += blockDim.x) {
+        if (!lc[j]) continue;
+        const uint32_t g = (first - 1) * CT_FAN + j;
+        atomicAdd(&ts.ncnt[g], lc[j]);
+        atomicMin((unsigned long long*)&ts.nmin[g], lmin[j]);
+        atomicMax((unsigned long long*)&ts.nmax[g], lmax[j]);
+    }
+}
+
+// One workgroup of 1024 threads: the entries of the nodes made in the last
+// round (one wave per node, four sub-cells per lane), then, unless `last`,
+// the next level of nodes among their sub-cells (heaviest first, over twice
+// the mean top-cell mass, within the node budget, splittable and not all one
+// value).  Sampled keys of an unsplit sub-cell whose keys are all one value
+// count as keys on repeated values.
+__global__ __launch_bounds__(1024) void k_tab_nodes(TabScratch ts, uint32_t ns, uint32_t* tab, uint32_t lvl) {
+    const bool last = lvl >= CT_DEPTH;  // the deepest level splits no further
+    static_assert(CT_FAN == 4 * 64, "four sub-cells per lane");
+    constexpr int PER = (CT_NODES * CT_FAN + 1023) / 1024;
+    __shared__ uint32_t wsum[16];
+    const uint32_t t = threadIdx.x, lane = t & 63u, wid = t >> 6;
+    const uint32_t first = ts.ctl[1], nlast = ts.ctl[0];
+    if (first > nlast) return;
+    for (uint32_t k = first + wid; k <= nlast; k += 16) {
+        const uint32_t par = ts.nparent[k];
+        const uint32_t lo = tab[par] & CT_CDF_MASK, hi = tab[par + 1] & CT_CDF_MASK;
+        uint32_t v[4], sum = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            v[q] = ts.ncnt[(k - 1) * CT_FAN + 4 * lane + q];
+            sum += v[q];
+        }
+        uint32_t incl = sum;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t y = __shfl_up(incl, d);
+            if (lane >= (uint32_t)d) incl += y;
+        }
+        const uint32_t total = __shfl(incl, 63);
+        uint64_t ex = incl - sum;
+        uint32_t* e = tab + ct_node(k);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            e[4 * lane + q] = lo + (uint32_t)(((uint64_t)(hi - lo) * ex) / (total ? total : 1u));
+            ex += v[q];
+        }
+        if (lane == 63) e[CT_FAN] = hi;
+    }
+    __syncthreads();
+    // candidates: sub-cell j of the new nodes (row-major), one count per slot
+    const uint32_t rows = (nlast - first + 1) * CT_FAN;
+    uint32_t cand[PER];
+    bool one[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
-        const uint32_t k = tid * PER + q;
-        if (k <= LEARN_KNOTS) cdf[k] = fmaxf(excl, v[q]);
+        const uint32_t j = q * 1024 + t;
+        cand[q] = 0;
+        one[q] = false;
+        if (j < rows) {
+            const uint32_t g = (first - 1) * CT_FAN + j;
+            const uint32_t k = first + j / CT_FAN;
+            const uint32_t a0 = ts.ncnt[g];
+            one[q] = a0 >= 2 && ts.nmin[g] == ts.nmax[g];
+            if (!last && !one[q] && ts.nw[k] >= 2 * CT_FAN_LOG) cand[q] = a0;
+        }
+    }
+    const uint32_t budget = CT_NODES - nlast;
+    const uint32_t thr = heaviest_threshold<PER>(cand, max(2u * (ns / CT_TOP), 1u), budget, wsum);
+    uint32_t npick = 0, mine = 0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) mine += cand[q] > thr;
+    const uint32_t r0 = block_excl_1024(mine, wsum, &npick);
+    uint32_t r = r0, dup = 0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const uint32_t j = q * 1024 + t;
+        if (j >= rows) continue;
+        const uint32_t k = first + j / CT_FAN, i = j % CT_FAN;
+        if (cand[q] > thr) {
+            const uint32_t id = nlast + 1 + r++;
+            const uint32_t at = ct_node(k) + i;
+            tab[at] |= id << CT_CDF_BITS;
+            ts.nparent[id] = at;
+            ts.nw[id] = ts.nw[k] - CT_FAN_LOG;
+            ts.noff[id] = ts.noff[k] + ((uint64_t)i << (ts.nw[k] - CT_FAN_LOG));
+        } else if (one[q]) {
+            dup += ts.ncnt[(first - 1) * CT_FAN + j];
+        }
+    }
+    if (dup) atomicAdd(&ts.ctl[2], dup);
+    for (uint32_t j = t; j < npick * (CT_FAN + 1); j += blockDim.x) tab[ct_node(nlast + 1) + j] = 0u;
+    __syncthreads();
+    if (t == 0) {
+        tab[3] = nlast + npick;
+        for (uint32_t w = 4 + lvl; w < 8; ++w) tab[w] = nlast + npick;
+        ts.ctl[1] = nlast + 1;
+        ts.ctl[0] = nlast + npick;
+    }
+}
+
+// Word index and fraction where the table's CDF (CT_CDF_BITS) reaches y,
+// and the offset of that position from the origin: a binary search in each
+// node on the way down (building time only).
+struct TabPos {
+    uint32_t at;
+    double f, off;
+};
+__device__ TabPos tab_inverse(const uint32_t* tab, uint32_t y) {
+    const uint32_t shift = tab[2];
+    uint32_t base = CT_OFF_TOP, n = CT_TOP, w = shift;
+    double off = 0.0;
+    uint32_t at = CT_OFF_TOP;
+    for (uint32_t lv = 0; lv <= CT_DEPTH; ++lv) {
+        uint32_t lo = 0, hi = n - 1;  // last entry with CDF <= y
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) / 2;
+            if ((tab[base + mid] & CT_CDF_MASK) <= y) lo = mid;
+            else hi = mid - 1;
+        }
+        at = base + lo;
+        const uint32_t cw = lv == 0 ? shift : w - CT_FAN_LOG;  // log2 of this entry's cell width
+        off += ldexp((double)lo, (int)cw);
+        w = cw;
+        const uint32_t child = tab[at] >> CT_CDF_BITS;
+        if (!child || lv == CT_DEPTH) break;
+        base = ct_node(child);
+        n = CT_FAN;
+    }
+    const uint32_t b0 = tab[at] & CT_CDF_MASK, b1 = tab[at + 1] & CT_CDF_MASK;
+    const double f = b1 > b0 ? fmin((double)(y - b0) / (double)(b1 - b0), 1.0) : 0.0;
+    return TabPos{at, f, off + f * ldexp(1.0, (int)w)};
+}
+
+// One workgroup: the network's M training points (features at evenly spaced
+// CDF values of the sample's table) and its initial parameters (the
+// piecewise-linear interpolation through 33 of them).
+__global__ __launch_bounds__(256) void k_tab_train_points(const TabScratch ts, const uint32_t* tab, uint32_t M, float* u,
+                                                          float* tg, float* params) {
+    const uint64_t span = ts.geo[1] - ts.geo[0];
+    const double inv = span ? 1.0 / (double)span : 0.0;
+    const double lead = (double)(ts.geo[0] - ((uint64_t)tab[0] | ((uint64_t)tab[1] << 32)));  // kmin - origin
+    auto feature_at = [&](double q) {  // feature of the key where the table's CDF is q in [0, 1]
+        const double yq = q * (double)CT_CDF_MASK;
+        const TabPos ps = tab_inverse(tab, (uint32_t)yq);
+        const double f = (ps.off - lead) * inv;
+        return (float)(f < 0.0 ? 0.0 : (f < 1.0 ? f : 1.0));
+    };
+    for (uint32_t i = threadIdx.x; i < M; i += blockDim.x) {
+        const double q = ((double)i + 0.5) / (double)M;
+        u[i] = feature_at(q);
+        tg[i] = (float)q;
+    }
+    __shared__ float kq[MLP_H + 1];
+    if (threadIdx.x <= MLP_H) kq[threadIdx.x] = feature_at((double)threadIdx.x / MLP_H);
+    __syncthreads();
+    if (threadIdx.x < MLP_H) {
+        const int j = threadIdx.x;
+        const float k0 = kq[j], k1 = kq[j + 1];
+        const float w = 1.0f / fmaxf(k1 - k0, 1e-6f);
+        params[j] = sqrtf(w);                        // p1
+        params[MLP_OFF_B1 + j] = -k0 * w;            // b1
+        params[MLP_OFF_B2 + j] = 0.0f;               // b2
+        params[MLP_OFF_P3 + j] = sqrtf(1.0f / MLP_H);
+        for (int i = 0; i < MLP_H; ++i) params[MLP_OFF_P2 + j * MLP_H + i] = (i == j) ? 1.0f : 0.02f;
+    }
+    if (threadIdx.x == 0) params[MLP_OFF_B3] = 0.0f;
+}
+
+// The second level's table: the first one's cells and nodes, the network's
+// CDF at every boundary, each value kept within a quarter of the smaller
+// adjacent cell's sampled mass, and within a quarter of a leaf's mass, around
+// the sample table's value (neighbouring bands cannot overlap), then made
+// non-decreasing inside each node and within the node's bounds.  A wider band
+// lets the network's error (its fit is coarser than the sample's resolution)
+// bend the CDF inside a root bucket, and the second level's leaves then lose
+// their balance: measured with a band of a quarter of the cell mass, 2^28
+// keys needed a third level.  One workgroup of CT_TOP threads; nodes level by level
+// (a node's id is larger than its parent's).
+__device__ __forceinline__ uint32_t wave_max_scan(uint32_t v) {  // inclusive, over the wave's lanes
+    const uint32_t lane = threadIdx.x & 63u;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(v, d);
+        if (lane >= (uint32_t)d) v = o > v ? o : v;
+    }
+    return v;
+}
+
+__global__ __launch_bounds__(CT_TOP) void k_tab_mlp(const uint32_t* tab1, uint32_t* tab2, const float* params,
+                                                    const TabScratch ts, uint32_t lf1) {
+    __shared__ float p[MLP_NP];
+    __shared__ uint32_t wmax[16];
+    const uint32_t t = threadIdx.x, lane = t & 63u, wid = t >> 6;
+    for (uint32_t i = t; i < MLP_NP; i += blockDim.x) p[i] = params[i];
+    for (uint32_t i = t; i < CT_OFF_TOP; i += blockDim.x) tab2[i] = tab1[i];
+    __syncthreads();
+    const uint64_t span = ts.geo[1] - ts.geo[0];
+    const double inv = span ? 1.0 / (double)span : 0.0;
+    const double lead = (double)(ts.geo[0] - ((uint64_t)tab1[0] | ((uint64_t)tab1[1] << 32)));  // kmin - origin
+    // a quarter of a leaf's mass: the second level splits a root bucket into
+    // at most FMAX leaves, so a boundary moved by less than that cannot make
+    // any leaf more than half a leaf larger
+    const double leaf4 = 0.25 * ldexp((double)CT_CDF_MASK, -(int)(lf1 + FMAX_LOG));
+    // network value at offset `off` from the origin, kept within the band
+    // around emp (m0, m1: the sampled masses of the cells on either side)
+    auto smooth = [&](double off, uint32_t emp, uint32_t m0, uint32_t m1) -> uint32_t {
+        const double f = (off - lead) * inv;
+        const float y = mlp_eval(p, (float)(f < 0.0 ? 0.0 : (f < 1.0 ? f : 1.0)));
+        const double band = fmin(0.25 * (double)(m0 < m1 ? m0 : m1), leaf4);
+        double v = (double)y * (double)CT_CDF_MASK;
+        v = fmin(fmax(v, (double)emp - band), (double)emp + band);
+        v = fmin(fmax(v, 0.0), (double)CT_CDF_MASK);
+        return (uint32_t)v;
+    };
+    // top boundaries, one per thread; entry CT_TOP is fixed
+    {
+        const uint32_t c = t, shift = tab1[2];
+        const uint32_t w0 = tab1[CT_OFF_TOP + c], e = w0 & CT_CDF_MASK;
+        const uint32_t m0 = c ? e - (tab1[CT_OFF_TOP + c - 1] & CT_CDF_MASK) : 0u;
+        const uint32_t m1 = (tab1[CT_OFF_TOP + c + 1] & CT_CDF_MASK) - e;
+        const uint32_t v = c ? smooth(ldexp((double)c, (int)shift), e, m0, m1) : 0u;
+        const uint32_t m = wave_max_scan(v);
+        if (lane == 63) wmax[wid] = m;
+        __syncthreads();
+        uint32_t before = 0;
+        for (uint32_t q = 0; q < wid; ++q) before = wmax[q] > before ? wmax[q] : before;
+        tab2[CT_OFF_TOP + c] = (w0 & ~CT_CDF_MASK) | (m > before ? m : before);
+        if (c == 0) tab2[CT_OFF_TOP + CT_TOP] = CT_CDF_MASK;
+    }
+    __syncthreads();
+    // nodes, one wave each, four entries per lane, level by level (header
+    // word 4 + l - 1: the last node of level l), so a node's parent is final
+    // before it
+    for (uint32_t lv = 0; lv < CT_DEPTH; ++lv) {
+    const uint32_t k_first = lv ? tab1[4 + lv - 1] + 1 : 1u, nn = tab1[4 + lv];
+    for (uint32_t k0 = k_first; k0 <= nn; k0 += 16) {
+        const uint32_t k = k0 + wid;
+        if (k <= nn) {
+            const uint32_t* e1 = tab1 + ct_node(k);
+            uint32_t* e2 = tab2 + ct_node(k);
+            const uint32_t par = ts.nparent[k];
+            const uint32_t lo = tab2[par] & CT_CDF_MASK, hi = tab2[par + 1] & CT_CDF_MASK;
+            const double o0 = (double)ts.noff[k];
+            const uint32_t sw = ts.nw[k] - CT_FAN_LOG;
+            uint32_t v[4], run = 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t i = 4 * lane + q;
+                const uint32_t e = e1[i] & CT_CDF_MASK;
+                const uint32_t m0 = i ? e - (e1[i - 1] & CT_CDF_MASK) : 0u, m1 = (e1[i + 1] & CT_CDF_MASK) - e;
+                uint32_t x = i ? smooth(o0 + ldexp((double)i, (int)sw), e, m0, m1) : lo;
+                x = x < lo ? lo : (x > hi ? hi : x);
+                run = x > run ? x : run;
+                v[q] = run;
+            }
+            const uint32_t m = wave_max_scan(run);
+            const uint32_t up = __shfl_up(m, 1);
+            const uint32_t before = lane ? up : 0u;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) e2[4 * lane + q] = (e1[4 * lane + q] & ~CT_CDF_MASK) | (v[q] > before ? v[q] : before);
+            if (lane == 0) e2[CT_FAN] = hi;
+        }
+        __syncthreads();
+    }
+    }
+}
+
+// The second level's local tables, one workgroup per root bucket b: the
+// bucket's key range [X_b, X_b+1) runs from where the sample's table crosses
+// b / F1 to where it crosses (b + 1) / F1; LT_CELLS cells of width 2^sh from
+// a multiple of 2^sh cover it; each boundary holds the network table's CDF
+// there, renormalised so that X_b maps to 0 and X_b+1 to 1 (clamped).
+__global__ __launch_bounds__(256) void k_tab_local(const uint32_t* tab1, const uint32_t* ntab, uint32_t lf1,
+                                                   const uint64_t* geo, uint32_t* loc) {
+    __shared__ uint64_t lo_off, hi_off;
+    const uint32_t b = blockIdx.x, F1 = 1u << lf1;
+    const uint64_t org = (uint64_t)tab1[0] | ((uint64_t)tab1[1] << 32);
+    if (threadIdx.x == 0) {
+        // the outer buckets end at the sampled extremes (the table's cells
+        // reach beyond them; keys past them go to the first or last cell)
+        const uint64_t y0 = ((uint64_t)b << CT_CDF_BITS) >> lf1, y1 = ((uint64_t)(b + 1) << CT_CDF_BITS) >> lf1;
+        lo_off = b ? (uint64_t)tab_inverse(tab1, (uint32_t)y0).off : geo[0] - org;
+        hi_off = b + 1 < F1 ? (uint64_t)tab_inverse(tab1, (uint32_t)y1).off : geo[1] - org;
+        if (hi_off < lo_off) hi_off = lo_off;
+    }
+    __syncthreads();
+    const uint64_t span = hi_off - lo_off;
+    uint32_t sh = 0;
+    while (sh < 56 && ((span >> sh) >= LT_CELLS - 1)) ++sh;  // one cell spare for the aligned origin
+    const uint64_t o = lo_off & ~((1ull << sh) - 1ull);
+    const uint64_t lim = ~0ull - org;  // offsets saturate at the largest encoded key
+    auto at = [&](uint64_t off) { return (uint64_t)table_cdf(ntab, org + (off > lim ? lim : off)); };
+    const uint64_t ya = at(lo_off), ye = at(hi_off);
+    uint32_t* lt = loc + (size_t)b * LT_WORDS;
+    if (threadIdx.x == 0) {
+        lt[0] = (uint32_t)(org + o);
+        lt[1] = (uint32_t)((org + o) >> 32);
+        lt[2] = sh;
+        lt[3] = 0;
+    }
+    for (uint32_t k = threadIdx.x; k <= LT_CELLS; k += blockDim.x) {
+        const uint64_t off = o + ((uint64_t)k << sh);
+        const uint64_t y = at(off < o ? lim : off);
+        uint32_t v;
+        if (y <= ya || ye <= ya) v = 0u;
+        else if (y >= ye) v = 0xFFFFFFFFu;
+        else v = (uint32_t)min<uint64_t>(((y - ya) << 32) / (ye - ya), 0xFFFFFFFFull);
+        lt[4 + k] = v;
     }
 }
 
@@ -413,34 +898,49 @@ s512_status build_learned_model(const T* a, size_t n, hipStream_t st, DeviceStat
         n, std::min<uint64_t>(std::max<uint64_t>((n * prm.sample_permille) / 1000, 64 * f1), 1u << 26));
     const uint32_t M = TRAIN_P;
     const int lf1 = std::min(std::max(1, leaves <= 1 ? 0 : 64 - __builtin_clzll(leaves - 1)), prm.max_logf);
-    // a feature value on more than `run` sample keys holds more than
-    // prm.imbalance times a first-level bucket's mean mass
-    const uint32_t run = (uint32_t)std::max<double>(1.0, std::ceil(prm.imbalance * (double)ns / (double)(1u << lf1)));
     SizeCounter sc;
     sc.take<K>(ns);
+    sc.take<uint64_t>(2);
+    sc.take<uint32_t>(CT_TOP);
+    sc.take<uint64_t>(CT_TOP);
+    sc.take<uint64_t>(CT_TOP);
+    sc.take<uint32_t>(CT_NODES * CT_FAN);
+    sc.take<uint64_t>(CT_NODES * CT_FAN);
+    sc.take<uint64_t>(CT_NODES * CT_FAN);
+    sc.take<uint32_t>(CT_NODES + 1);
+    sc.take<uint64_t>(CT_NODES + 1);
+    sc.take<uint32_t>(CT_NODES + 1);
+    sc.take<uint32_t>(4);
+    sc.take<uint32_t>(CT_WORDS);
+    sc.take<uint32_t>(CT_WORDS);
+    sc.take<uint32_t>((size_t)FMAX * LT_WORDS);
     sc.take<float>(M);
     sc.take<float>(M);
     sc.take<float>(MLP_NP);
     sc.take<float>(1);
-    sc.take<uint64_t>(2);
-    sc.take<uint64_t>(LEARN_KNOTS + 1);
-    for (int i = 0; i < 4; ++i) sc.take<float>(LEARN_KNOTS + 1);
-    sc.take<uint32_t>(1);
     s512_status rc;
     if ((rc = ds.learn.reserve(sc.off, st))) return rc;
     Arena ar{(char*)ds.learn.ptr, 0};
     K* smp = ar.take<K>(ns);
+    TabScratch ts;
+    ts.geo = ar.take<uint64_t>(2);
+    ts.cnt = ar.take<uint32_t>(CT_TOP);
+    ts.cmin = ar.take<uint64_t>(CT_TOP);
+    ts.cmax = ar.take<uint64_t>(CT_TOP);
+    ts.ncnt = ar.take<uint32_t>(CT_NODES * CT_FAN);
+    ts.nmin = ar.take<uint64_t>(CT_NODES * CT_FAN);
+    ts.nmax = ar.take<uint64_t>(CT_NODES * CT_FAN);
+    ts.nparent = ar.take<uint32_t>(CT_NODES + 1);
+    ts.noff = ar.take<uint64_t>(CT_NODES + 1);
+    ts.nw = ar.take<uint32_t>(CT_NODES + 1);
+    ts.ctl = ar.take<uint32_t>(4);
+    uint32_t* tab1 = ar.take<uint32_t>(CT_WORDS);
+    uint32_t* ntab = ar.take<uint32_t>(CT_WORDS);                 // the network's table
+    uint32_t* loc = ar.take<uint32_t>((size_t)FMAX * LT_WORDS);   // the root buckets' local tables
     float* u = ar.take<float>(M);
     float* t = ar.take<float>(M);
     float* params = ar.take<float>(MLP_NP);
     float* loss = ar.take<float>(1);
-    uint64_t* geo = ar.take<uint64_t>(2);
-    uint64_t* key = ar.take<uint64_t>(LEARN_KNOTS + 1);
-    float* ecdf_raw = ar.take<float>(LEARN_KNOTS + 1);
-    float* ecdf = ar.take<float>(LEARN_KNOTS + 1);
-    float* cdf_raw = ar.take<float>(LEARN_KNOTS + 1);
-    float* cdf = ar.take<float>(LEARN_KNOTS + 1);
-    uint32_t* flag = ar.take<uint32_t>(1);
     if (!ds.lstream) {
         // default priority; does not synchronise with the null stream
         S512_HIP(hipStreamCreateWithFlags(&ds.lstream, hipStreamNonBlocking));
@@ -454,28 +954,31 @@ s512_status build_learned_model(const T* a, size_t n, hipStream_t st, DeviceStat
         (void)hipEventCreate(&e1);
         (void)hipEventRecord(e0, st);
     }
-    k_learn_sample<T><<<(ns + 255) / 256, 256, 0, st>>>(a, n, ns, prm.seed ^ 0x1EA2ull, smp);
+    const uint32_t g = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((ns + 16383) / 16384, 256));
+    uint32_t run = LEARN_RUN_MAX;
+    while (run > 1 && ns / run < 512ull << lf1) run >>= 1;
+    k_learn_sample<T><<<(ns + 255) / 256, 256, 0, st>>>(a, n, ns, run, prm.seed ^ 0x1EA2ull, smp);
     S512_CHECK_LAUNCH();
-    // the sample is sorted by this library's own engine (lock already held),
-    // with leaves of up to 8192 keys: for a few million keys one partition
-    // level plus a short second one for the buckets above a tile
-    EngineParams sp = default_params(sizeof(K) == 8);
-    sp.target_leaf = 8192;
-    if ((rc = sort_engine_locked<K, false>(smp, nullptr, ns, st, sp, nullptr, false, ds))) return rc;
-    k_learn_prep<K><<<1, 256, 0, st>>>(smp, ns, M, u, t, params, geo);
+    k_tab_init<<<1, 256, 0, st>>>(ts);
     S512_CHECK_LAUNCH();
-    S512_HIP(hipMemsetAsync(flag, 0, sizeof(uint32_t), st));
-    k_point_mass<K><<<(ns + 255) / 256, 256, 0, st>>>(smp, ns, run, geo, flag);
+    k_tab_range<K><<<g, 256, 0, st>>>(smp, ns, ts.geo);
     S512_CHECK_LAUNCH();
-    // the first level's knots: the sample's own CDF
-    k_knots<K><<<(LEARN_KNOTS + 1 + 255) / 256, 256, 0, st>>>(smp, ns, nullptr, geo, key, ecdf_raw);
+    k_tab_hist<K><<<g, 256, 0, st>>>(smp, ns, ts);
     S512_CHECK_LAUNCH();
-    k_knot_scan<<<1, 1024, 0, st>>>(ecdf_raw, ecdf);
+    k_tab_top<<<1, CT_TOP, 0, st>>>(ts, ns, tab1);
+    S512_CHECK_LAUNCH();
+    for (uint32_t lvl = 1; lvl <= CT_DEPTH; ++lvl) {
+        k_tab_nodehist<K><<<g, 256, 0, st>>>(smp, ns, ts, tab1);
+        S512_CHECK_LAUNCH();
+        k_tab_nodes<<<1, 1024, 0, st>>>(ts, ns, tab1, lvl);
+        S512_CHECK_LAUNCH();
+    }
+    k_tab_train_points<<<1, 256, 0, st>>>(ts, tab1, M, u, t, params);
     S512_CHECK_LAUNCH();
     if ((rc = ds.pinned2.reserve(4096, st))) return rc;
     char* h = (char*)ds.pinned2.ptr;
-    S512_HIP(hipMemcpyAsync(h, geo, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-    S512_HIP(hipMemcpyAsync(h + 128, flag, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    S512_HIP(hipMemcpyAsync(h, ts.geo, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    S512_HIP(hipMemcpyAsync(h + 128, ts.ctl + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     if (stats) (void)hipEventRecord(e1, st);
     S512_HIP(hipEventRecord(ds.lready, st));
     S512_HIP(hipStreamSynchronize(st));
@@ -483,11 +986,12 @@ s512_status build_learned_model(const T* a, size_t n, hipStream_t st, DeviceStat
     memcpy(g2, h, sizeof(g2));
     uint32_t pm;
     memcpy(&pm, h + 128, sizeof(pm));
-    out->ecdf = LearnedModelDev{key, ecdf, smp, ns, (uint32_t)sizeof(K)};
-    out->knots = LearnedModelDev{key, cdf, smp, ns, (uint32_t)sizeof(K)};
+    out->tabs = LearnedModelDev{tab1, loc};
     out->params = params;
     out->degenerate = g2[1] <= g2[0];
-    out->point_mass = pm != 0;
+    // over an eighth of the sample on repeated values: the table cannot split
+    // them, the sampled splitters' equality buckets can
+    out->point_mass = pm > ns / 8;
     out->lf1 = (uint32_t)lf1;
     out->training = false;
     out->trained = ds.ltrained;
@@ -503,14 +1007,14 @@ s512_status build_learned_model(const T* a, size_t n, hipStream_t st, DeviceStat
     }
     if (out->degenerate || out->point_mass) return S512_OK;
     // the network: trained on the training stream while the caller runs the
-    // first level; its knots (same keys, its own CDF) serve the second level
+    // first level on the sample's table; its table serves the second level
     hipStream_t ls = ds.lstream;
     S512_HIP(hipStreamWaitEvent(ls, ds.lready, 0));
     k_mlp_train<<<1, TRAIN_P, 0, ls>>>(params, u, t, M, prm.train_steps, prm.learn_rate, loss, nullptr);
     S512_CHECK_LAUNCH();
-    k_knots<K><<<(LEARN_KNOTS + 1 + 255) / 256, 256, 0, ls>>>(smp, ns, params, geo, nullptr, cdf_raw);
+    k_tab_mlp<<<1, CT_TOP, 0, ls>>>(tab1, ntab, params, ts, (uint32_t)lf1);
     S512_CHECK_LAUNCH();
-    k_knot_scan<<<1, 1024, 0, ls>>>(cdf_raw, cdf);
+    k_tab_local<<<1u << lf1, 256, 0, ls>>>(tab1, ntab, (uint32_t)lf1, ts.geo, loc);
     S512_CHECK_LAUNCH();
     S512_HIP(hipMemcpyAsync(ds.lpinned.ptr, loss, sizeof(float), hipMemcpyDeviceToHost, ls));
     S512_HIP(hipEventRecord(ds.ltrained, ls));

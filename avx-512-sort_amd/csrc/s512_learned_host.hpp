@@ -7,24 +7,23 @@
 namespace s512 {
 
 struct LearnedModel {
-    LearnedModelDev ecdf;   // knots with the sample's own CDF: the first level
-    LearnedModelDev knots;  // knots with the trained network's CDF: the second level
+    LearnedModelDev tabs;   // CDF tables: the sample's (first level), the network's (second level)
     const float* params;    // device parameters (trained once `trained` completes)
     bool degenerate;        // all sampled keys equal: no model
-    bool point_mass;        // one feature value holds more than the imbalance
+    bool point_mass;        // one key value holds more than the imbalance
                             // limit's share of the sample: rejected
     uint32_t lf1;           // log2 of the first-level fan-out the model serves
     bool training;          // training launched on ds.lstream
-    hipEvent_t trained;     // recorded on ds.lstream after the trained knots
+    hipEvent_t trained;     // recorded on ds.lstream after the network's table
     const float* loss_host; // final training loss (pinned), valid after `trained`
 };
 
-// Samples ~prm.sample_permille/1000 of a (device array of n keys), sorts the
-// sample with the engine and computes the sample-CDF knots on stream st
-// (synchronised); then, unless the sample shows a point mass, launches the
-// network's training and its knots on ds.lstream, which runs while the
-// caller partitions the first level.  Caller holds ds.mu and must wait for
-// `trained` before the call returns (or before using `knots`).
+// Samples ~prm.sample_permille/1000 of a (device array of n keys) and builds
+// the sample's CDF table from histograms on stream st (synchronised); then,
+// unless the sample shows a point mass, launches the network's training and
+// its table on ds.lstream, which runs while the caller partitions the first
+// level.  Caller holds ds.mu and must wait for `trained` before the call
+// returns (or before using tabs.tab2, the root buckets' local tables).
 template <class T>
 s512_status build_learned_model(const T* a, size_t n, hipStream_t st, DeviceState& ds, const EngineParams& prm,
                                 LearnedModel* out, s512_stats* stats);

@@ -291,10 +291,10 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
             p.lshift = 0;
             p.lbase = 0;
             if (use_learned && (segs[i].aux & 1u)) {
-                // the model places the splitters: of the root level (quantile
-                // range [0, 1), 2^lf1 buckets), then of each root bucket j
-                // (range [j, j + 1) / 2^lf1, fan-out from the bucket's size)
-                p.mode = 1;
+                // the model's CDF table places every key: in the root level
+                // (2^lf1 buckets), then inside each root bucket j (fan-out
+                // from the bucket's size)
+                p.mode = 2;
                 if (level == 1) {
                     lf = (int)lf1;
                 } else {
@@ -304,7 +304,7 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
             }
             p.logf = (uint32_t)lf;
             const uint32_t F = 1u << lf, B = 2 * F;
-            p.nsample = p.mode ? F : std::min<uint32_t>(smax, prm.oversample * F);
+            p.nsample = p.mode ? 0u : std::min<uint32_t>(smax, prm.oversample * F);
             p.chunk = chunk;
             p.nchunks = (uint32_t)((p.size + chunk - 1) / chunk);
             max_nchunks = std::max(max_nchunks, p.nchunks);
@@ -317,12 +317,22 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
             p.flags = 0;
             p.sample_base = (uint64_t)i * smax;
             sumF += F;
-            sumG += guide_words(F);
+            if (!p.mode) sumG += guide_words(F);
             histN += (uint64_t)p.nchunks * B;
             sumB += B;
             maxB = std::max<uint64_t>(maxB, B);
             for (uint32_t c = 0; c < p.nchunks; ++c) crs.push_back(ChunkRef{i, c});
         }
+        // learned segments: after the guides, one copy of the root's CDF table
+        // (level 1) or of the root buckets' local tables (level 2)
+        const uint64_t tab_off = sumG;
+        bool level_learned = false;
+        for (auto& p : ps)
+            if (p.mode) {
+                p.guide_base = tab_off + (p.lshift ? (uint64_t)p.lbase * LT_WORDS : 0ull);
+                level_learned = true;
+            }
+        if (level_learned) sumG += level == 1 ? CT_WORDS : ((size_t)LT_WORDS << lf1);
         const uint32_t tc = (uint32_t)crs.size();
         const uint64_t copy_cap = sumB + total / COPY_PIECE + 1;
         // ---- arena layout: the uploaded prefix first
@@ -417,10 +427,11 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
             pf.end();
         }
         if (any_learned) {
+            // the first level uses the sample's table, the second the network's
             pf.begin(S512_PHASE_SAMPLE);
-            k_model_splitters<K><<<m, 256, 0, st>>>(d_ps, d_cnt + 12, level == 1 ? lstate.l1 : lstate.l2, lstate.l1,
-                                                     d_smp);
-            S512_CHECK_LAUNCH();
+            const size_t words = level == 1 ? CT_WORDS : ((size_t)LT_WORDS << lf1);
+            S512_HIP(hipMemcpyAsync(d_guide + tab_off, level == 1 ? lstate.tabs.tab1 : lstate.tabs.tab2,
+                                    words * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
             pf.end();
         }
         pf.begin(S512_PHASE_TREE);
@@ -488,6 +499,15 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
         if (counts[14]) {
             set_error("plan lists overflowed at level %d", level);
             return S512_ERR_INTERNAL;
+        }
+        if (any_learned && level == 2 && stats) {
+            uint64_t lk = 0, lb = 0;  // keys and buckets of the model's second level
+            for (auto& p : ps)
+                if (p.mode) {
+                    lk += p.size;
+                    lb += 1ull << p.logf;
+                }
+            if (lk) stats->learned_leaf_max_ratio = (float)((double)counts64[10] * (double)lb / (double)lk);
         }
         // level 1 saw every key: fp64 leaves may use min/max when no key is NaN or -0.0
         if (level == 1) f64_fast = !KV && std::is_same<T, double>::value && prm.f64_minmax && counts[11] == 0;
@@ -636,13 +656,12 @@ s512_status sort_engine_locked(T* a, int32_t* va, size_t n, hipStream_t st, cons
     LevelState ls;
     ls.use_learned = use_learned;
     ls.lf1 = lf1;
-    ls.l1 = model.ecdf;
-    ls.l2 = model.knots;
+    ls.tabs = model.tabs;
     ls.trained = model.trained;
     ls.loss_host = model.loss_host;
     ls.max_loss = prm.max_loss;
     ls.pending = use_learned && model.training;
-    rc = run_levels<T, KV>(a, va, b, vb, n, segs, st, prm, stats, pf, ds, model.knots, ls);
+    rc = run_levels<T, KV>(a, va, b, vb, n, segs, st, prm, stats, pf, ds, model.tabs, ls);
     if (stats) stats->engine = S512_ENGINE_TWO_PASS;
     if (rc == S512_OK && stats && model.training && ls.pending) {
         // the sort finished within one level: report the loss all the same

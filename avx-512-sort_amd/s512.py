@@ -153,13 +153,15 @@ class Stats(ctypes.Structure):
                 ("launches", ctypes.c_uint32 * NPHASE), ("keys", ctypes.c_uint64 * NPHASE),
                 ("classifier", ctypes.c_int32), ("learned_fallback", ctypes.c_int32),
                 ("learned_levels", ctypes.c_int32), ("learned_max_bucket_ratio", ctypes.c_float),
-                ("model_loss", ctypes.c_float), ("engine", ctypes.c_int32), ("region_overflow", ctypes.c_int32)]
+                ("model_loss", ctypes.c_float), ("engine", ctypes.c_int32), ("region_overflow", ctypes.c_int32),
+                ("learned_leaf_max_ratio", ctypes.c_float)]
 
     def as_dict(self):
         return {"levels": self.levels, "tiles": self.tiles, "classifier": self.classifier,
                 "engine": self.engine, "region_overflow": self.region_overflow,
                 "learned_fallback": self.learned_fallback, "learned_levels": self.learned_levels,
                 "learned_max_bucket_ratio": self.learned_max_bucket_ratio, "model_loss": self.model_loss,
+                "learned_leaf_max_ratio": self.learned_leaf_max_ratio,
                 "phases": {PHASES[i]: {"ms": self.kernel_ms[i], "launches": self.launches[i], "keys": self.keys[i]}
                            for i in range(NPHASE) if self.launches[i] or self.keys[i]}}
 

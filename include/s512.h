@@ -263,6 +263,8 @@ typedef struct s512_stats {
                                            partition levels ran (0: no level) */
     int32_t region_overflow;            /* 1: a one-pass root region overflowed and
                                            the sort ran with two passes        */
+    float learned_leaf_max_ratio;       /* largest second-level bucket placed by the
+                                           learned model / their mean          */
 } s512_stats;
 
 /* Sort512::Sort with options and statistics (device pointers). */
