@@ -22,6 +22,7 @@
 #include "s512_common.hpp"
 #include "s512_tile.hpp"
 #include "s512_learned.hpp"
+#include <type_traits>
 
 namespace s512 {
 
@@ -163,10 +164,13 @@ __device__ __forceinline__ uint32_t lower_bound_lds(const K* srt, uint32_t n, K 
 // Load a segment's classifier into LDS: splitters + guide table.
 constexpr int LDS_TABLE_WORDS = GMAX;
 static_assert(CT_WORDS <= LDS_TABLE_WORDS, "the learned CDF table fits the guide's LDS");
-template <class K, int NT>
+// LRN: a build for levels whose segments are all learned (mode 2; the host
+// launches it for those levels only).  Each build carries one classifier, so
+// neither pays registers (and occupancy) for the other's code.
+template <class K, int NT, bool LRN>
 __device__ __forceinline__ Guide load_classifier(const PartSeg& s, const K* trees, const uint32_t* guides, K* lsrt,
                                                  uint32_t* lguide) {
-    if (s.mode == 2) {
+    if constexpr (LRN) {
         // the learned model's CDF table (s512_learned.hpp), no splitters: the
         // root's tree table, or below the root the bucket's local table
         const uint32_t words = s.lshift ? LT_WORDS : CT_WORDS;
@@ -251,12 +255,13 @@ __device__ __forceinline__ void classify_batch(const K* srt, const uint32_t* gd,
 }
 
 
-// Buckets of keys of segment s: splitters and guide (modes 0, 1), or the
-// learned CDF table in O(1) per key (mode 2: ordinary buckets only).
-template <class K, int NB>
+// Buckets of keys of segment s: splitters and guide (modes 0, 1), or in the
+// learned build the model's CDF table, O(1) per key (mode 2: ordinary buckets
+// only).
+template <class K, int NB, bool LRN>
 __device__ __forceinline__ void classify_keys(const PartSeg& s, const K* srt, const uint32_t* gd, const Guide& g,
                                               const K (&x)[NB], uint32_t (&bk)[NB]) {
-    if (s.mode == 2) {
+    if constexpr (LRN) {
         if (s.lshift) {
 #pragma unroll
             for (int u = 0; u < NB; ++u) bk[u] = local_bucket(gd, s.logf, x[u]);
@@ -267,11 +272,11 @@ __device__ __forceinline__ void classify_keys(const PartSeg& s, const K* srt, co
         classify_batch<K, NB>(srt, gd, g, x, bk);
     }
 }
-template <class K>
+template <bool LRN, class K>
 __device__ __forceinline__ uint32_t classify_key(const PartSeg& s, const K* srt, const uint32_t* gd, const Guide& g,
                                                  K x) {
-    if (s.mode == 2) return s.lshift ? local_bucket(gd, s.logf, x) : table_bucket(gd, s.logf, x);
-    return classify(srt, gd, g, x);
+    if constexpr (LRN) return s.lshift ? local_bucket(gd, s.logf, x) : table_bucket(gd, s.logf, x);
+    else return classify(srt, gd, g, x);
 }
 
 // ---------------------------------------------------------------- tile kernels
@@ -757,7 +762,7 @@ __device__ __forceinline__ void wave_rank_many(uint32_t* cnt, uint32_t* dummy, c
 // scatter does not classify again.  Batches of CNT_P*CNT_U keys alternate
 // between two register buffers: the next batch is in flight while this one is
 // classified, with no register copies that would wait for it.
-template <class T, int CNT_P, int CNT_U, int NB>
+template <class T, int CNT_P, int CNT_U, int NB, bool LRN>
 __global__ __launch_bounds__(CNT_P) void k_count(const PartSeg* ps, const ChunkRef* cr, const uint32_t* nchunk,
                                                  const T* a, const T* b, const typename KeyTraits<T>::K* trees,
                                                  const uint32_t* guides, uint32_t* hist, uint32_t* special,
@@ -774,7 +779,7 @@ __global__ __launch_bounds__(CNT_P) void k_count(const PartSeg* ps, const ChunkR
     const PartSeg s = ps[c.seg];
     const uint32_t F = 1u << s.logf, B = 2 * F;
     const uint32_t t = threadIdx.x;
-    const Guide g = load_classifier<K, CNT_P>(s, trees, guides, lsrt, lguide);
+    const Guide g = load_classifier<K, CNT_P, LRN>(s, trees, guides, lsrt, lguide);
     for (uint32_t j = t; j < B; j += CNT_P) lh[j] = 0;
     __syncthreads();
     const T* src = (s.buf ? b : a) + s.begin;
@@ -809,7 +814,7 @@ __global__ __launch_bounds__(CNT_P) void k_count(const PartSeg* ps, const ChunkR
                 uint32_t kb[NB];
 #pragma unroll
                 for (int u = 0; u < NB; ++u) xb[u] = x[u0 + u];
-                classify_keys<K, NB>(s, lsrt, lguide, g, xb, kb);
+                classify_keys<K, NB, LRN>(s, lsrt, lguide, g, xb, kb);
 #pragma unroll
                 for (int u = 0; u < NB; ++u) bk[u0 + u] = kb[u];
             }
@@ -827,7 +832,7 @@ __global__ __launch_bounds__(CNT_P) void k_count(const PartSeg* ps, const ChunkR
             for (int u = 0; u < CNT_U; ++u) {
                 const uint64_t i = base + (uint64_t)u * CNT_P + t;
                 if (i < hi) {
-                    const uint32_t bk = classify_key(s, lsrt, lguide, g, x[u]);
+                    const uint32_t bk = classify_key<LRN>(s, lsrt, lguide, g, x[u]);
                     if (special) odd |= nan_or_negzero(x[u]);
                     atomicAdd(&lh[bk], 1u);
                     if (oid) oid[i] = (uint16_t)bk;
@@ -852,14 +857,19 @@ __global__ __launch_bounds__(CNT_P) void k_count(const PartSeg* ps, const ChunkR
     }
 }
 
-// Count variants: workgroup threads, keys per thread per batch, and keys
-// classified together.  grid = chunks (the kernel loops over the device count).
+// Key types the learned model covers: only their kernels get a learned build.
 template <class T>
-inline void launch_count(int shape, uint32_t grid, hipStream_t st, const PartSeg* ps, const ChunkRef* cr,
-                         const uint32_t* nchunk, const T* a, const T* b, const typename KeyTraits<T>::K* trees,
-                         const uint32_t* guides, uint32_t* hist, uint32_t* special, uint16_t* ids) {
+constexpr bool learnable_key() { return std::is_same<T, double>::value || std::is_same<T, int32_t>::value; }
+
+// Count variants: workgroup threads, keys per thread per batch, and keys
+// classified together; grid = chunks (the kernel loops over the device count).
+// lrn: every segment of the level is learned.
+template <class T, bool LRN>
+inline void launch_count_as(int shape, uint32_t grid, hipStream_t st, const PartSeg* ps, const ChunkRef* cr,
+                            const uint32_t* nchunk, const T* a, const T* b, const typename KeyTraits<T>::K* trees,
+                            const uint32_t* guides, uint32_t* hist, uint32_t* special, uint16_t* ids) {
 #define S512_COUNT(PP, UU, NN) \
-    k_count<T, PP, UU, NN><<<grid, PP, 0, st>>>(ps, cr, nchunk, a, b, trees, guides, hist, special, ids)
+    k_count<T, PP, UU, NN, LRN><<<grid, PP, 0, st>>>(ps, cr, nchunk, a, b, trees, guides, hist, special, ids)
     switch (shape) {
         case 1: S512_COUNT(256, 16, 4); break;
         case 2: S512_COUNT(256, 8, 8); break;
@@ -868,6 +878,18 @@ inline void launch_count(int shape, uint32_t grid, hipStream_t st, const PartSeg
         default: S512_COUNT(256, 16, 8); break;
     }
 #undef S512_COUNT
+}
+template <class T>
+inline void launch_count(int shape, uint32_t grid, hipStream_t st, const PartSeg* ps, const ChunkRef* cr,
+                         const uint32_t* nchunk, const T* a, const T* b, const typename KeyTraits<T>::K* trees,
+                         const uint32_t* guides, uint32_t* hist, uint32_t* special, uint16_t* ids, bool lrn = false) {
+    if constexpr (learnable_key<T>()) {
+        if (lrn) {
+            launch_count_as<T, true>(shape, grid, st, ps, cr, nchunk, a, b, trees, guides, hist, special, ids);
+            return;
+        }
+    }
+    launch_count_as<T, false>(shape, grid, st, ps, cr, nchunk, a, b, trees, guides, hist, special, ids);
 }
 
 // ---------------------------------------------------------------- column scan
@@ -985,7 +1007,7 @@ struct ScatterLds {
     uint32_t dcount;
 };
 
-template <class T, bool KV, bool IDS, int P, int E, bool WC>
+template <class T, bool KV, bool IDS, int P, int E, bool WC, bool LRN = false>
 __global__ __launch_bounds__(P) void k_scatter(const PartSeg* ps, const ChunkRef* cr, T* a, T* b, int32_t* va,
                                                int32_t* vb, const typename KeyTraits<T>::K* trees,
                                                const uint32_t* guides, const uint32_t* hist, const uint64_t* bstart,
@@ -1014,7 +1036,7 @@ __global__ __launch_bounds__(P) void k_scatter(const PartSeg* ps, const ChunkRef
     const uint32_t B = 2u << s.logf;
     const uint32_t t = threadIdx.x, lane = t & 63u, wid = t >> 6;
     Guide g{0, 0, 0, 0, 0, 0};
-    if constexpr (!IDS) g = load_classifier<K, P>(s, trees, guides, sm.lsrt, sm.lguide);
+    if constexpr (!IDS) g = load_classifier<K, P, LRN>(s, trees, guides, sm.lsrt, sm.lguide);
     const uint32_t* h = hist + s.hist_base + (uint64_t)c.idx * B;
     for (uint32_t j = t; j < BMAX; j += P) {
         sm.cursor[j] = j < B ? (uint32_t)bstart[s.bkt_base + j] + h[j] : 0u;
@@ -1070,13 +1092,13 @@ __global__ __launch_bounds__(P) void k_scatter(const PartSeg* ps, const ChunkRef
                 uint32_t kb[CLS_NB];
 #pragma unroll
                 for (int u = 0; u < CLS_NB; ++u) kx[u] = Tr::enc(cur.x[e0 + u]);
-                classify_keys<K, CLS_NB>(s, sm.lsrt, sm.lguide, g, kx, kb);
+                classify_keys<K, CLS_NB, LRN>(s, sm.lsrt, sm.lguide, g, kx, kb);
 #pragma unroll
                 for (int u = 0; u < CLS_NB; ++u) bk[e0 + u] = kb[u];
             }
         } else {
 #pragma unroll
-            for (int e = 0; e < E; ++e) bk[e] = classify_key(s, sm.lsrt, sm.lguide, g, Tr::enc(cur.x[e]));
+            for (int e = 0; e < E; ++e) bk[e] = classify_key<LRN>(s, sm.lsrt, sm.lguide, g, Tr::enc(cur.x[e]));
         }
         if (cnt == ROUND) {
             wave_rank_many<E>(sm.rh, sm.dummy, bk, lr);
@@ -1515,7 +1537,7 @@ struct ScatterLbLds {
     T lbuf[FMAX * LB];
 };
 
-template <class T, bool IDS, int P, int E>
+template <class T, bool IDS, int P, int E, bool LRN = false>
 __global__ __launch_bounds__(P) void k_scatter_lb(const PartSeg* ps, const ChunkRef* cr, T* a, T* b,
                                                   const typename KeyTraits<T>::K* trees, const uint32_t* guides,
                                                   const uint32_t* hist, const uint64_t* bstart,
@@ -1545,7 +1567,7 @@ __global__ __launch_bounds__(P) void k_scatter_lb(const PartSeg* ps, const Chunk
     const uint64_t dph = (uint64_t)(uintptr_t)dst / sizeof(T);
     const uint16_t* isrc = IDS ? ids + s.begin : nullptr;
     Guide g{0, 0, 0, 0, 0, 0};
-    if constexpr (!IDS) g = load_classifier<K, P>(s, trees, guides, sm.lsrt, sm.lguide);
+    if constexpr (!IDS) g = load_classifier<K, P, LRN>(s, trees, guides, sm.lsrt, sm.lguide);
     const uint32_t* h = hist + s.hist_base + (uint64_t)c.idx * B;
     for (uint32_t j = t; j < BMAX; j += P) {
         const uint32_t cur = j < B ? (uint32_t)bstart[s.bkt_base + j] + h[j] : 0u;
@@ -1602,13 +1624,13 @@ __global__ __launch_bounds__(P) void k_scatter_lb(const PartSeg* ps, const Chunk
                 uint32_t kb[CLS_NB];
 #pragma unroll
                 for (int u = 0; u < CLS_NB; ++u) kx[u] = Tr::enc(cur.x[e0 + u]);
-                classify_keys<K, CLS_NB>(s, sm.lsrt, sm.lguide, g, kx, kb);
+                classify_keys<K, CLS_NB, LRN>(s, sm.lsrt, sm.lguide, g, kx, kb);
 #pragma unroll
                 for (int u = 0; u < CLS_NB; ++u) bk[e0 + u] = kb[u];
             }
         } else {
 #pragma unroll
-            for (int e = 0; e < E; ++e) bk[e] = classify_key(s, sm.lsrt, sm.lguide, g, Tr::enc(cur.x[e]));
+            for (int e = 0; e < E; ++e) bk[e] = classify_key<LRN>(s, sm.lsrt, sm.lguide, g, Tr::enc(cur.x[e]));
         }
         if (cnt == ROUND) {
             wave_rank_many<E>(sm.rh, sm.dummy, bk, lr);
@@ -1732,10 +1754,30 @@ inline void launch_scatter(int shape, bool use_ids, uint32_t grid, hipStream_t s
                            const ChunkRef* cr, T* a, T* b, int32_t* va, int32_t* vb,
                            const typename KeyTraits<T>::K* trees, const uint32_t* guides, const uint32_t* hist,
                            const uint64_t* bstart, const uint32_t* nchunk,
-                           const uint16_t* ids) {
+                           const uint16_t* ids, bool lrn = false) {
 #define S512_SCATTER(I, PP, EE, W)                                                                              \
     k_scatter<T, KV, I, PP, EE, W><<<grid, PP, 0, st>>>(ps, cr, a, b, va, vb, trees, guides, hist, bstart, nchunk, \
                                                         ids)
+#define S512_SCATTER_L(PP, EE)                                                                                  \
+    k_scatter<T, false, false, PP, EE, false, true><<<grid, PP, 0, st>>>(ps, cr, a, b, va, vb, trees, guides, hist, \
+                                                                          bstart, nchunk, ids)
+    if constexpr (!KV && learnable_key<T>()) {
+        // a level of learned segments that classifies again in the scatter: the
+        // learned build of the same shape (with stored ids nothing classifies)
+        if (lrn && !use_ids) {
+            constexpr int LE = sizeof(T) == 8 ? 8 : 16;
+            switch (shape) {
+                case SCATTER_LB:
+                    k_scatter_lb<T, false, 512, LE, true><<<grid, 512, 0, st>>>(ps, cr, a, b, trees, guides, hist,
+                                                                                  bstart, nchunk, ids);
+                    break;
+                case 1: S512_SCATTER_L(256, 16); break;
+                case 2: S512_SCATTER_L(1024, 8); break;
+                default: S512_SCATTER_L(512, 16); break;
+            }
+            return;
+        }
+    }
     if constexpr (!KV) {
         if (shape == SCATTER_FL && use_ids) {
             // persistent: about one workgroup per CU, each on a contiguous range of chunks
@@ -1775,6 +1817,7 @@ inline void launch_scatter(int shape, bool use_ids, uint32_t grid, hipStream_t s
             default: S512_SCATTER(false, 512, 16, false); break;
         }
     }
+#undef S512_SCATTER_L
 #undef S512_SCATTER
 }
 

@@ -173,7 +173,7 @@ __global__ __launch_bounds__(256) void k_cap_count(const PartSeg* ps, const T* a
     __shared__ uint32_t lh[FMAX];
     const PartSeg s = ps[blockIdx.y];
     const uint32_t F = 1u << s.logf;
-    const Guide g = load_classifier<K, 256>(s, trees, guides, lsrt, lguide);
+    const Guide g = load_classifier<K, 256, false>(s, trees, guides, lsrt, lguide);
     for (uint32_t j = threadIdx.x; j < F; j += 256) lh[j] = 0;
     __syncthreads();
     const T* src = (s.buf ? b : a) + s.begin;
@@ -325,7 +325,7 @@ __global__ __launch_bounds__(P, WPE) void k_onepass(OpArgs A, const T* src, T* d
     if (wk0 >= total) return;  // uniform
     ChunkRef c = A.cr[wk0];
     PartSeg s = A.ps[c.seg];
-    Guide g = load_classifier<K, P>(s, trees, guides, sm.lsrt, sm.lguide);
+    Guide g = load_classifier<K, P, false>(s, trees, guides, sm.lsrt, sm.lguide);
     K xb[E];
     int32_t xv[KV ? E : 1];
     op_load_chunk<T, KV, P, E>(xb, xv, src, vsrc, c, s, t);

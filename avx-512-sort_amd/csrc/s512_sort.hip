@@ -408,6 +408,11 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
         const uint64_t seed = prm.seed + 0x9E37ull * (uint64_t)level;
         bool any_split = false, any_learned = false;
         for (auto& p : ps) (p.mode ? any_learned : any_split) = true;
+        if (any_learned && any_split) {
+            // the count and scatter builds carry one classifier each
+            set_error("level %d mixes learned and sampled segments", level);
+            return S512_ERR_INTERNAL;
+        }
         if (any_split) {
             pf.begin(S512_PHASE_SAMPLE);
             k_sample<T><<<m, 256, 0, st>>>(d_ps, d_cnt + 12, a, b, d_smp, smax, seed);
@@ -441,7 +446,7 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
         pf.begin(S512_PHASE_COUNT);
         // NaN / -0.0 keys are looked for in the first level only (it sees every key)
         launch_count<T>(prm.count_shape, tc, st, d_ps, d_cr, d_cnt + 13, a, b, d_tree, d_guide, d_hist,
-                        level == 1 ? d_cnt + 11 : nullptr, d_ids);
+                        level == 1 ? d_cnt + 11 : nullptr, d_ids, any_learned);
         S512_CHECK_LAUNCH();
         pf.end();
         pf.begin(S512_PHASE_SCAN);
@@ -476,7 +481,7 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
         }
         pf.begin(S512_PHASE_SCATTER);
         launch_scatter<T, KV>(prm.scatter_shape, d_ids != nullptr, tc, st, d_ps, d_cr, a, b, va, vb, d_tree,
-                              d_guide, d_hist, d_bstart, d_cnt + 13, d_ids);
+                              d_guide, d_hist, d_bstart, d_cnt + 13, d_ids, any_learned);
         S512_CHECK_LAUNCH();
         pf.end();
         pf.begin(S512_PHASE_PLAN);
