@@ -50,4 +50,9 @@ template <class T, bool KV>
 s512_status sort_engine_locked(T* a, int32_t* va, size_t n, hipStream_t st, const EngineParams& prm,
                                s512_stats* stats, bool profile, DeviceState& ds);
 
+// Sorted runs of width w of a[0, n) merged into one (merge path, through the
+// device's scratch buffer); caller holds ds.mu.
+template <class T>
+s512_status merge_runs_locked(T* a, size_t n, uint64_t w, hipStream_t st, DeviceState& ds);
+
 }  // namespace s512

@@ -138,13 +138,39 @@ static s512_status merge_inplace_dev(T* a, size_t size, size_t center, hipStream
     return S512_OK;
 }
 
+// Sorted runs of width w (the last one may be shorter) of a[0, n) merged,
+// round after round, into one; ds.data holds the other buffer of each round.
+// Caller holds ds.mu.
+template <class T>
+s512_status merge_runs_locked(T* a, size_t n, uint64_t w, hipStream_t st, DeviceState& ds) {
+    if (n <= 1 || w >= n) return S512_OK;
+    const uint64_t ntiles = (n + MRG_D - 1) / MRG_D;
+    s512_status rc;
+    if ((rc = ds.data.reserve(n * sizeof(T), st))) return rc;
+    if ((rc = ds.arena.reserve(2 * ntiles * sizeof(uint64_t), st))) return rc;
+    uint64_t* split = (uint64_t*)ds.arena.ptr;
+    T* src = a;
+    T* dst = (T*)ds.data.ptr;
+    for (; w < n; w *= 2) {
+        if ((rc = merge_round<T>(src, dst, MergeShape{n, w, 0}, split, st))) return rc;
+        std::swap(src, dst);
+    }
+    if (src != a) S512_HIP(hipMemcpyAsync(a, src, n * sizeof(T), hipMemcpyDeviceToDevice, st));
+    S512_HIP(hipStreamSynchronize(st));
+    return S512_OK;
+}
+template s512_status merge_runs_locked<int32_t>(int32_t*, size_t, uint64_t, hipStream_t, DeviceState&);
+template s512_status merge_runs_locked<double>(double*, size_t, uint64_t, hipStream_t, DeviceState&);
+template s512_status merge_runs_locked<int64_t>(int64_t*, size_t, uint64_t, hipStream_t, DeviceState&);
+template s512_status merge_runs_locked<uint64_t>(uint64_t*, size_t, uint64_t, hipStream_t, DeviceState&);
+template s512_status merge_runs_locked<uint32_t>(uint32_t*, size_t, uint64_t, hipStream_t, DeviceState&);
+
 // Merge sort: bitonic tiles of the largest tile class, then merge-path rounds.
 template <class T>
 static s512_status merge_sort_dev(T* a, size_t n, hipStream_t st) {
     using K = typename KeyTraits<T>::K;
     if (n <= 1) return S512_OK;
     if (!a) { set_error("null array"); return S512_ERR_INVALID; }
-    if (n >= (1ull << 32)) { set_error("n = %zu exceeds 2^32-1 keys", n); return S512_ERR_TOO_LARGE; }
     int dev;
     s512_status rc = current_device(&dev);
     if (rc) return rc;

@@ -439,10 +439,12 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
                                     words * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
             pf.end();
         }
-        pf.begin(S512_PHASE_TREE);
-        k_build_tree<K><<<m, FMAX, 0, st>>>(d_ps, d_cnt + 12, d_smp, smax, d_tree, d_guide);
-        S512_CHECK_LAUNCH();
-        pf.end();
+        if (any_split) {
+            pf.begin(S512_PHASE_TREE);
+            k_build_tree<K><<<m, FMAX, 0, st>>>(d_ps, d_cnt + 12, d_smp, smax, d_tree, d_guide);
+            S512_CHECK_LAUNCH();
+            pf.end();
+        }
         pf.begin(S512_PHASE_COUNT);
         // NaN / -0.0 keys are looked for in the first level only (it sees every key)
         launch_count<T>(prm.count_shape, tc, st, d_ps, d_cr, d_cnt + 13, a, b, d_tree, d_guide, d_hist,
@@ -559,13 +561,42 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
     return S512_OK;
 }
 
+// Arrays of 2^32 keys or more: the partition levels keep segment-relative
+// offsets in 32 bits, so such an array is sorted as pieces of SORT_PIECE keys
+// (each one ordinary engine sort), which merge-path rounds then merge.
+constexpr uint64_t SORT_PIECE = 1ull << 31;
+
+template <class T>
+static s512_status sort_pieces_locked(T* a, size_t n, hipStream_t st, const EngineParams& prm, s512_stats* stats,
+                                      bool profile, DeviceState& ds) {
+    s512_stats ps;
+    for (uint64_t b = 0; b < n; b += SORT_PIECE) {
+        const uint64_t len = std::min<uint64_t>(SORT_PIECE, n - b);
+        s512_status rc = sort_engine_locked<T, false>(a + b, nullptr, len, st, prm, stats ? &ps : nullptr, profile, ds);
+        if (rc) return rc;
+        if (!stats) continue;
+        if (b == 0) {
+            *stats = ps;  // the first piece's model and fallback figures
+            continue;
+        }
+        stats->levels = std::max(stats->levels, ps.levels);
+        stats->tiles += ps.tiles;
+        for (int j = 0; j < S512_NPHASE; ++j) {
+            stats->kernel_ms[j] += ps.kernel_ms[j];
+            stats->launches[j] += ps.launches[j];
+            stats->keys[j] += ps.keys[j];
+        }
+    }
+    return merge_runs_locked<T>(a, n, SORT_PIECE, st, ds);
+}
+
 template <class T, bool KV>
 s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const EngineParams& prm, s512_stats* stats,
                         bool profile) {
     if (stats) memset(stats, 0, sizeof(*stats));
     if (n <= 1) return S512_OK;
-    if (n >= (1ull << 32)) {
-        set_error("n = %zu exceeds the supported maximum of 2^32-1 keys", n);
+    if (KV && n >= (1ull << 32)) {
+        set_error("n = %zu: key/value sorts take fewer than 2^32 pairs", n);
         return S512_ERR_TOO_LARGE;
     }
     int dev;
@@ -573,6 +604,9 @@ s512_status sort_engine(T* a, int32_t* va, size_t n, hipStream_t st, const Engin
     if (rc) return rc;
     DeviceState& ds = device_state(dev);
     std::lock_guard<std::mutex> lk(ds.mu);
+    if constexpr (!KV) {
+        if (n >= (1ull << 32)) return sort_pieces_locked<T>(a, n, st, prm, stats, profile, ds);
+    }
     return sort_engine_locked<T, KV>(a, va, n, st, prm, stats, profile, ds);
 }
 

@@ -42,14 +42,25 @@ sys.path.insert(0, HERE)
 
 KNOBS = {
     "count_variant": [1, 2, 3, 4, 5],
-    "scatter_variant": [1, 2, 3, 4, 5, 6, 7, 8],
+    "scatter_variant": [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11],
     "tile_variant": [1, 2, 3],
     "target_leaf": [512, 1024, 2048, 4096],
     "max_fanout_log2": [7, 8, 9],
 }
 START = {"count_variant": 3, "scatter_variant": 1, "tile_variant": 3, "target_leaf": 1024, "max_fanout_log2": 9}
 KIND_ID = {"i32": 0, "f64": 1, "kv": 2, "seg_i32": 3, "seg_f64": 4}
-SCATTER_SHAPES = 4  # include/s512.h: scatter_variant = 1 + shape + 4 * reclassify
+SCATTER_SHAPES = 4  # include/s512.h: scatter_variant = 1 + shape + 4 * reclassify (1..8)
+# 9, 10: the line-buffered scatter (table shape 4) with stored ids / classifying;
+# 11: the full-line scatter (table shape 5, stored ids)
+SCATTER_SPECIAL = {9: (4, 1), 10: (4, 0), 11: (5, 1)}
+
+
+def scatter_columns(variant):
+    """(scatter_shape, ids) table columns of a scatter_variant."""
+    if variant in SCATTER_SPECIAL:
+        return SCATTER_SPECIAL[variant]
+    sv = variant - 1
+    return sv % SCATTER_SHAPES, 1 if sv < SCATTER_SHAPES else 0
 
 # (kind, lg_lo, lg_hi, representative n): n in [2^lg_lo, 2^lg_hi) uses the entry
 CLASSES = [
@@ -223,9 +234,9 @@ def write_inc(rows, path):
              "static const TuneEntry kTune[] = {"]
     for r in rows:
         o = r["options"]
-        sv = o["scatter_variant"] - 1
+        shape, ids = scatter_columns(o["scatter_variant"])
         lines.append(f"    {{{KIND_ID[r['kind']]}, {r['lg_lo']}, {r['lg_hi']}, {o['count_variant'] - 1}, "
-                     f"{sv % SCATTER_SHAPES}, {1 if sv < SCATTER_SHAPES else 0}, {o['tile_variant'] - 1}, "
+                     f"{shape}, {ids}, {o['tile_variant'] - 1}, "
                      f"{o['target_leaf']}, {o['max_fanout_log2']}}},  // n ~ {r['n']}: {r['ms']:.3f} ms summed")
     lines.append("};")
     with open(path, "w") as f:

@@ -87,6 +87,8 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=1 << 28)  # the full 268,435,456-key uniform input
     ap.add_argument("--cpu-budget", type=float, default=150.0,
                     help="seconds of CPU-baseline work beyond the headline sample (items past it are skipped)")
+    ap.add_argument("--option", action="append", default=[], metavar="NAME=VALUE",
+                    help="s512_options field for the headline sorts (repeatable), e.g. scatter_variant=11")
     ap.add_argument("--workload", default="plain", choices=list(WORKLOADS),
                     help="plain: the 256M fp64 headline (default); the others time one secondary workload "
                          "and print its own JSON line")
@@ -109,22 +111,32 @@ def progress(msg):
     print(f"[bench {time.time() - _T0:6.1f}s] {msg}", file=sys.stderr, flush=True)
 
 
+HASH_SLICE = 1 << 28  # keys per slice of the checks (bounds their temporaries)
+
+
 def multiset_hash(t):
     """Order-independent hash of the bit patterns (sum of a 64-bit mix)."""
     import torch
-    x = t.view(torch.int64) if t.dtype == torch.float64 else t.to(torch.int64)
-    x = x * -7046029254386353131  # 0x9E3779B97F4A7C15 as int64 (wraps)
-    x = x ^ (x >> 29)
-    x = x * -4658895280553007687  # 0xBF58476D1CE4E5B9
-    x = x ^ (x >> 32)
-    return int(x.sum().item())
+    h = 0
+    for i in range(0, t.numel(), HASH_SLICE):
+        s = t[i:i + HASH_SLICE]
+        x = s.view(torch.int64) if s.dtype == torch.float64 else s.to(torch.int64)
+        x = x * -7046029254386353131  # 0x9E3779B97F4A7C15 as int64 (wraps)
+        x = x ^ (x >> 29)
+        x = x * -4658895280553007687  # 0xBF58476D1CE4E5B9
+        x = x ^ (x >> 32)
+        h += int(x.sum().item())
+    return ((h + (1 << 63)) % (1 << 64)) - (1 << 63)  # the sum modulo 2^64, as one slice would wrap it
 
 
 def is_sorted(t):
     import torch
-    if t.numel() < 2:
-        return True
-    return bool(torch.all(t[1:] >= t[:-1]).item())
+    n = t.numel()
+    for i in range(0, n - 1, HASH_SLICE):
+        s = t[i:min(n, i + HASH_SLICE + 1)]  # slices overlap by one key
+        if not bool(torch.all(s[1:] >= s[:-1]).item()):
+            return False
+    return True
 
 
 # ------------------------------------------------------------------ host CPU
@@ -613,7 +625,8 @@ def learned_comparison(s512, n, reps=3):
                     raise RuntimeError(f"learned and splitter sorts differ on {kind}")
                 if name == "learned":
                     info = {"fallback": int(st.learned_fallback), "levels_by_model": int(st.learned_levels),
-                            "max_bucket_ratio": round(float(st.learned_max_bucket_ratio), 3)}
+                            "max_bucket_ratio": round(float(st.learned_max_bucket_ratio), 3),
+                            "leaf_max_ratio": round(float(st.learned_leaf_max_ratio), 3)}
         sp, le = median(times["splitters"]), median(times["learned"])
         out[kind] = {"splitters_ms": sp, "learned_ms": le, "speedup": sp / le, **info}
     del src, work
@@ -749,6 +762,7 @@ def bench_learned(args):
                 d = st["s"].as_dict()
                 r[name].update({"classifier": d["classifier"], "fallback": d["learned_fallback"],
                                 "max_bucket_ratio": round(d["learned_max_bucket_ratio"], 3),
+                                "leaf_max_ratio": round(d["learned_leaf_max_ratio"], 3),
                                 "model_loss": d["model_loss"]})
         r["learned_vs_splitters"] = r["splitters"]["ms"] / r["learned"]["ms"]
         res[kind] = r
@@ -829,6 +843,28 @@ def bench_plain_i32(args, traffic=None):
                  {"workload": f"plain sort, {n} int32 keys x {len(dists)} distributions ({','.join(dists)})"},
                  {"verified": ok, "per_distribution_ms": per, "roofline": roof,
                   "whole_sort_uniform": _whole(traffic, "i32_sort", per["uniform"], n, 8)})
+
+
+def bench_large(args):
+    """The plain sort past 2^32 keys: 4,294,967,296 + 4,099 int32 keys, uniform
+    over the full range (pieces of 2^31 keys sorted by the partition levels,
+    then merge-path rounds).  --n overrides the size."""
+    import torch
+    import s512
+    n = args.n if args.n != 268_435_456 else (1 << 32) + 4099
+    dev = torch.device("cuda:0")
+    src = torch.empty(n, dtype=torch.int32, device=dev)
+    work = torch.empty_like(src)
+    s512.fill(src, "uniform", seed=3)
+    h0 = multiset_hash(src)
+    ms = _timed(lambda: s512.sort_ex(work), lambda: work.copy_(src), args.steps, args.warmup)
+    ok = is_sorted(work) and multiset_hash(work) == h0
+    del src, work
+    torch.cuda.empty_cache()
+    return _line("sorted keys/s, int32 past 2^32 keys", "keys/s", n, ms, args.steps, args.warmup, "int32",
+                 "synthetic (device-generated SplitMix64, uniform)",
+                 {"workload": f"plain sort, {n} int32 keys (uniform), pieces of 2^31 keys + merge-path rounds"},
+                 {"verified": ok})
 
 
 def bench_partition(args, traffic=None):
@@ -1014,7 +1050,8 @@ def bench_topk(args):
 WORKLOADS = {"plain": None, "plain-i32": bench_plain_i32, "host": bench_host, "wide": bench_wide,
              "topk": bench_topk, "kv-zipf": bench_kv_zipf, "segsort-i32": lambda a: bench_segsort(a, "int32"),
              "segsort-f64": lambda a: bench_segsort(a, "float64"), "learned-1g": bench_learned,
-             "partition": bench_partition, "merge": bench_merge, "smallsort": bench_smallsort}
+             "partition": bench_partition, "merge": bench_merge, "smallsort": bench_smallsort,
+             "large": bench_large}
 
 
 def _summary(line):
@@ -1073,6 +1110,16 @@ def main():
     work = torch.empty(n, dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
     hashes = {d: multiset_hash(pristine[d]) for d in dists}
+    chosen = dict(kv.split("=", 1) for kv in args.option)
+
+    def options():
+        """Library defaults unless --option set a field."""
+        o = s512.Options()
+        for k, v in chosen.items():
+            if not hasattr(o, k):
+                raise SystemExit(f"--option: s512_options has no field {k!r}")
+            setattr(o, k, int(v))
+        return o
 
     # ---- timed steps: the production call, HIP events around each sort only
     ev0 = torch.cuda.Event(enable_timing=True)
@@ -1091,7 +1138,7 @@ def main():
         for d in dists:
             work.copy_(pristine[d])
             ev0.record()
-            st = s512.sort_ex(work)
+            st = s512.sort_ex(work, options())
             ev1.record()
             ev1.synchronize()
             if not timed:
@@ -1113,7 +1160,7 @@ def main():
         for d in dists:
             work.copy_(pristine[d])
             ev0.record()
-            st = s512.sort_ex(work, profile=True)
+            st = s512.sort_ex(work, options(), profile=True)
             ev1.record()
             ev1.synchronize()
             prof_ms.append(ev0.elapsed_time(ev1))
@@ -1146,7 +1193,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic (device-generated SplitMix64 / Box-Muller; five distributions)",
         "config": {"workload": f"plain sort, {n} fp64 keys x {len(dists)} distributions ({','.join(dists)})",
-                   "keys_per_step": keys_step, "classifier": "sampled splitters"},
+                   "keys_per_step": keys_step, "classifier": "sampled splitters",
+                   "options": chosen or "library defaults"},
         "roofline": roof,
         "whole_sort": _whole(traffic, "f64_sort", uni, n, 16) if uni else None,
         "effective_hbm_frac": (16.0 * keys_step / (t_step / 1e3) / 1e9) / HBM_PEAK_GBS,

@@ -21,7 +21,12 @@
  *     are accepted and placed by their bit pattern (negative NaNs first,
  *     positive NaNs last); the reference leaves NaN undefined.
  *   - n <= 1 is a no-op (the reference underflows size-1 at n == 0).
- *   - Sizes are limited to n < 2^32 keys per call.
+ *   - Sizes are size_t, as the reference's IndexType (sort512.hpp:5960).
+ *     Plain sorts and the merge sort take any n that fits in device memory
+ *     (2^32 keys or more: pieces of 2^31 keys, each sorted by the partition
+ *     levels, then merged by merge-path rounds; about 2n keys of scratch).
+ *     Key/value sorts and segmented sorts take n < 2^32; argsort and the
+ *     8-byte-value sorts n < 2^31 (their permutation is int32).
  *   - Calls on one device are serialised by the library (it keeps one
  *     grow-only workspace per device).
  */

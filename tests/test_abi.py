@@ -36,8 +36,8 @@ def test_host_validation_without_gpu(s512):
     assert b"null" in lib.s512_last_error()
     assert lib.s512kv_sort_i32(None, None, 5) == 1
     assert lib.s512_sort_f64_dev(None, 7, None) == 1
-    # beyond the supported size
-    assert lib.s512_sort_f64_dev(ctypes.c_void_p(16), 1 << 33, None) == 5
+    # beyond the supported size (key/value sorts: fewer than 2^32 pairs)
+    assert lib.s512kv_sort_i32_dev(ctypes.c_void_p(16), ctypes.c_void_p(16), 1 << 33, None) == 5
     # partition: null bound pointer, empty range
     assert lib.s512_partition_i32(None, 0, 5, 0, None) == 1
     b = ctypes.c_size_t(99)

@@ -5,8 +5,11 @@ record of the tuning run."""
 import json
 import os
 import re
+import sys
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "avx-512-sort_amd"))
+from s512_tune import scatter_columns  # noqa: E402
 INC = os.path.join(ROOT, "avx-512-sort_amd", "csrc", "s512_tune_table.inc")
 JSON = os.path.join(ROOT, "avx-512-sort_amd", "tune_table.json")
 
@@ -32,7 +35,8 @@ def test_table_covers_every_kind_and_size():
 
 def test_table_values_valid():
     for kind, lo, hi, cnt, sc, ids, tile, leaf, logf in _rows():
-        assert 0 <= cnt < 5 and 0 <= sc < 4 and ids in (0, 1) and 0 <= tile < 3, (kind, lo)
+        assert 0 <= cnt < 5 and 0 <= sc < 6 and ids in (0, 1) and 0 <= tile < 3, (kind, lo)
+        assert sc != 5 or ids == 1, (kind, lo)  # the full-line scatter reads stored ids
         assert 64 <= leaf <= 1 << 16 and 1 <= logf <= 9, (kind, lo)
 
 
@@ -45,5 +49,5 @@ def test_table_matches_tuning_record():
         o = c["options"]
         assert r[:3] == [kinds[c["kind"]], c["lg_lo"], c["lg_hi"]]
         assert r[3] == o["count_variant"] - 1 and r[6] == o["tile_variant"] - 1
-        assert r[4] == (o["scatter_variant"] - 1) % 4 and r[5] == int(o["scatter_variant"] <= 4)
+        assert (r[4], r[5]) == scatter_columns(o["scatter_variant"])
         assert r[7] == o["target_leaf"] and r[8] == o["max_fanout_log2"]
