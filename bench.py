@@ -570,18 +570,25 @@ def _kernel_traffic(traffic, op, prefix):
         return None
     f = w = 0.0
     cnt = 0
+    names = set()
     for kn, kd in d["kernels"].items():
-        if kn.split("s512::")[-1].startswith(prefix):
+        short = kn.split("s512::")[-1]
+        if short.startswith(prefix):
             f += kd["fetch_per_dispatch"] * kd["dispatches"]
             w += kd["write_per_dispatch"] * kd["dispatches"]
             cnt += kd["dispatches"]
+            names.add(short.split("<")[0])
     if not cnt:
         return None
-    return {"traffic": (f + w) / cnt, "fetch_bytes": f / cnt, "write_bytes": w / cnt, "dispatches": cnt}
+    return {"traffic": (f + w) / cnt, "fetch_bytes": f / cnt, "write_bytes": w / cnt, "dispatches": cnt,
+            "kernels": sorted(names)}
 
 
-# kernel-name prefixes of the sort's phases, per key type
-_PHASE_KERNEL = {t: {"scatter": "k_scatter<", "count": "k_count<", "tile": f"k_tile_sort<{t}", "copy": "k_copy_back"}
+# kernel-name prefixes of the sort's phases, per key type (the scatter phase
+# runs k_scatter, or the line-buffered / full-line kernels the tuned defaults
+# or options select)
+_PHASE_KERNEL = {t: {"scatter": ("k_scatter<", "k_scatter_fl<", "k_scatter_lb<"), "count": "k_count<",
+                     "tile": f"k_tile_sort<{t}", "copy": "k_copy_back"}
                  for t in ("double", "int")}
 
 
@@ -805,6 +812,8 @@ def _roofline(ph_ms, ph_l, ph_k, key_bytes, traffic_of=None):
         out["traffic"] = traffic["traffic"]
         out["traffic_detail"] = traffic
         out["measured_frac"] = traffic["traffic"] / (per_launch_ms / 1e3) / 1e9 / HBM_PEAK_GBS
+        if len(traffic.get("kernels", ())) == 1:
+            out["kernel"] = traffic["kernels"][0]  # the kernel the counter runs saw in this phase
     return out
 
 
