@@ -11,8 +11,11 @@ namespace s512 {
 
 // State of the learned partitioner across the level loop: the model's CDF
 // tables place the keys of the root level (2^lf1 buckets) and of its buckets.
+struct LearnedModel;
 struct LevelState {
     bool use_learned = false;
+    const LearnedModel* model = nullptr;  // the learned model (its rejection flags)
+    bool check_root = false;               // the model's rejection test is still due (first level)
     uint32_t lf1 = 0;
     LearnedModelDev tabs{nullptr, nullptr};  // the root's CDF table, the root buckets' local tables
     hipEvent_t trained = nullptr;            // tabs.tab2 is ready once this event completes

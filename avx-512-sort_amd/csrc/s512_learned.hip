@@ -954,6 +954,7 @@ s512_status build_learned_model(const T* a, size_t n, hipStream_t st, DeviceStat
         (void)hipEventCreate(&e1);
         (void)hipEventRecord(e0, st);
     }
+    uint64_t* flags = (uint64_t*)((char*)ds.lpinned.ptr + 64);  // kmin, kmax, repeated-value keys
     const uint32_t g = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((ns + 16383) / 16384, 256));
     uint32_t run = LEARN_RUN_MAX;
     while (run > 1 && ns / run < 512ull << lf1) run >>= 1;
@@ -975,37 +976,20 @@ s512_status build_learned_model(const T* a, size_t n, hipStream_t st, DeviceStat
     }
     k_tab_train_points<<<1, 256, 0, st>>>(ts, tab1, M, u, t, params);
     S512_CHECK_LAUNCH();
-    if ((rc = ds.pinned2.reserve(4096, st))) return rc;
-    char* h = (char*)ds.pinned2.ptr;
-    S512_HIP(hipMemcpyAsync(h, ts.geo, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-    S512_HIP(hipMemcpyAsync(h + 128, ts.ctl + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    S512_HIP(hipMemcpyAsync(flags, ts.geo, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    S512_HIP(hipMemcpyAsync(flags + 2, ts.ctl + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     if (stats) (void)hipEventRecord(e1, st);
     S512_HIP(hipEventRecord(ds.lready, st));
-    S512_HIP(hipStreamSynchronize(st));
-    uint64_t g2[2];
-    memcpy(g2, h, sizeof(g2));
-    uint32_t pm;
-    memcpy(&pm, h + 128, sizeof(pm));
     out->tabs = LearnedModelDev{tab1, loc};
     out->params = params;
-    out->degenerate = g2[1] <= g2[0];
-    // over an eighth of the sample on repeated values: the table cannot split
-    // them, the sampled splitters' equality buckets can
-    out->point_mass = pm > ns / 8;
+    out->flags_host = flags;
+    out->ns = ns;
     out->lf1 = (uint32_t)lf1;
     out->training = false;
     out->trained = ds.ltrained;
     out->loss_host = (const float*)ds.lpinned.ptr;
-    if (stats) {
-        float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        stats->kernel_ms[S512_PHASE_TRAIN] += ms;
-        stats->launches[S512_PHASE_TRAIN] += 1;
-        stats->keys[S512_PHASE_TRAIN] += ns;
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-    }
-    if (out->degenerate || out->point_mass) return S512_OK;
+    out->ev0 = e0;
+    out->ev1 = e1;
     // the network: trained on the training stream while the caller runs the
     // first level on the sample's table; its table serves the second level
     hipStream_t ls = ds.lstream;

@@ -226,7 +226,6 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
     const int ncls = TC::nclass();
     bool& use_learned = lstate.use_learned;
     const uint32_t lf1 = lstate.lf1;
-    bool learned_checked = false;
     std::vector<PartSeg> ps;
     std::vector<ChunkRef> crs;
     const uint32_t smax = 8192;
@@ -251,7 +250,7 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
             S512_HIP(hipEventSynchronize(lstate.trained));
             const float loss = *lstate.loss_host;
             if (stats) stats->model_loss = loss;
-            if (!std::isfinite(loss) || loss > lstate.max_loss) {
+            if (use_learned && (!std::isfinite(loss) || loss > lstate.max_loss)) {
                 use_learned = false;
                 if (stats) {
                     stats->learned_fallback = 1;
@@ -356,7 +355,10 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
         if ((rc = ds.pinned.reserve(up_bytes, st))) return rc;
         // pinned2: the level's counts (first 4 KiB), then the next level's segments
         const uint64_t pre_next = std::min<uint64_t>(sumB, 4096);  // segments read with the counts
-        if ((rc = ds.pinned2.reserve(4096 + std::max<size_t>(sumB, 1) * sizeof(Seg), st))) return rc;
+        // (+ the learned root level's bucket sizes, for the model's rejection test)
+        const size_t root_sizes_off = (4096 + std::max<size_t>(sumB, 1) * sizeof(Seg) + 63) & ~(size_t)63;
+        const bool root_check = lstate.check_root && level == 1;
+        if ((rc = ds.pinned2.reserve(root_sizes_off + (root_check ? sumB * sizeof(uint64_t) : 0), st))) return rc;
 
         Arena ar{(char*)ds.arena.ptr, 0}, hp{(char*)ds.pinned.ptr, 0};
         PartSeg* d_ps = ar.take<PartSeg>(m);
@@ -458,29 +460,6 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
         k_bucket_scan<<<m, BMAX, 0, st>>>(d_ps, d_cnt + 12, d_bsize, d_bstart);
         S512_CHECK_LAUNCH();
         pf.end();
-        if (any_learned && level == 1 && !learned_checked) {
-            // learned root level: fall back to sampled splitters when the
-            // model leaves an ordinary bucket far above the mean (nothing has
-            // moved yet; equality buckets are finished and do not count)
-            learned_checked = true;
-            const uint32_t B1 = 2u << lf1;
-            if ((rc = ds.pinned2.reserve(B1 * sizeof(uint64_t), st))) return rc;
-            uint64_t* hb = (uint64_t*)ds.pinned2.ptr;
-            S512_HIP(hipMemcpyAsync(hb, d_bsize, B1 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-            S512_HIP(hipStreamSynchronize(st));
-            uint64_t mx = 0;
-            for (uint32_t j = 0; j < B1; j += 2) mx = std::max(mx, hb[j]);
-            const double mean = (double)n / (double)(1u << lf1);
-            if (stats) stats->learned_max_bucket_ratio = (float)((double)mx / mean);
-            if ((double)mx > prm.imbalance * mean) {
-                if (stats) stats->learned_fallback = 1;
-                use_learned = false;
-                segs.assign(1, Seg{0, n, 0, 0});
-                level = 0;
-                continue;
-            }
-            if (stats) stats->learned_levels = 2;
-        }
         pf.begin(S512_PHASE_SCATTER);
         launch_scatter<T, KV>(prm.scatter_shape, d_ids != nullptr, tc, st, d_ps, d_cr, a, b, va, vb, d_tree,
                               d_guide, d_hist, d_bstart, d_cnt + 13, d_ids, any_learned);
@@ -498,6 +477,9 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
         // the first segments of the next level come back with the counts: one
         // host round trip per level instead of two
         S512_HIP(hipMemcpyAsync(h_next, pl.next, pre_next * sizeof(Seg), hipMemcpyDeviceToHost, st));
+        uint64_t* h_sizes = (uint64_t*)((char*)ds.pinned2.ptr + root_sizes_off);
+        if (root_check)
+            S512_HIP(hipMemcpyAsync(h_sizes, d_bsize, sumB * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
         S512_HIP(hipStreamSynchronize(st));
         uint32_t counts[32];
         uint64_t counts64[16];
@@ -506,6 +488,28 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
         if (counts[14]) {
             set_error("plan lists overflowed at level %d", level);
             return S512_ERR_INTERNAL;
+        }
+        if (root_check) {
+            // the learned model's rejection test, now that the stream has
+            // passed its flags and the root level's counts: a degenerate
+            // sample, repeated values it cannot split, or an ordinary bucket
+            // far above the mean.  A rejected model leaves the next level to
+            // sampled splitters; the root level's buckets are ordered key
+            // ranges all the same.
+            lstate.check_root = false;
+            const LearnedModel& md = *lstate.model;
+            uint64_t mx = 0;
+            for (uint64_t j = 0; j < sumB; j += 2) mx = std::max(mx, h_sizes[j]);
+            const double mean = (double)n / (double)(1u << lf1);
+            const bool degenerate = md.flags_host[1] <= md.flags_host[0];
+            if (stats) stats->learned_max_bucket_ratio = (float)((double)mx / mean);
+            if (learned_model_rejected(md) || (double)mx > prm.imbalance * mean) {
+                use_learned = false;
+                if (stats && !degenerate) stats->learned_fallback = 1;
+                if (stats) stats->learned_levels = 1;
+            } else if (stats) {
+                stats->learned_levels = 2;
+            }
         }
         if (any_learned && level == 2 && stats) {
             uint64_t lk = 0, lb = 0;  // keys and buckets of the model's second level
@@ -666,11 +670,12 @@ s512_status sort_engine_locked(T* a, int32_t* va, size_t n, hipStream_t st, cons
         if (prm.learned) {
             if ((rc = build_learned_model<T>(a, n, st, ds, prm, &model, stats))) return rc;
             if (model.training) train_wait.ev = model.trained;
-            // rejected outright: a point mass the network cannot split (a
-            // network that fails to train is caught before the second level)
-            const bool ok = !model.degenerate && !model.point_mass;
-            if (!model.degenerate && !ok && stats) stats->learned_fallback = 1;
-            use_learned = ok;
+            // the first level runs on the sample's table at once; whether the
+            // model is rejected (a degenerate sample, repeated values it cannot
+            // split, a bucket far above the mean) is read with that level's
+            // counts, and a rejected model leaves the second level to sampled
+            // splitters (a network that fails to train as well)
+            use_learned = true;
             lf1 = model.lf1;
         }
     }
@@ -694,6 +699,8 @@ s512_status sort_engine_locked(T* a, int32_t* va, size_t n, hipStream_t st, cons
     std::vector<Seg> segs{Seg{0, n, 0, use_learned ? 1u : 0u}};
     LevelState ls;
     ls.use_learned = use_learned;
+    ls.model = use_learned ? &model : nullptr;
+    ls.check_root = use_learned;
     ls.lf1 = lf1;
     ls.tabs = model.tabs;
     ls.trained = model.trained;
@@ -702,6 +709,18 @@ s512_status sort_engine_locked(T* a, int32_t* va, size_t n, hipStream_t st, cons
     ls.pending = use_learned && model.training;
     rc = run_levels<T, KV>(a, va, b, vb, n, segs, st, prm, stats, pf, ds, model.tabs, ls);
     if (stats) stats->engine = S512_ENGINE_TWO_PASS;
+    if (model.ev0) {
+        // the sample and its table, timed for the statistics (the stream has
+        // passed them: every level ends with a synchronisation)
+        float ms = 0.f;
+        if (rc == S512_OK && stats && hipEventElapsedTime(&ms, model.ev0, model.ev1) == hipSuccess) {
+            stats->kernel_ms[S512_PHASE_TRAIN] += ms;
+            stats->launches[S512_PHASE_TRAIN] += 1;
+            stats->keys[S512_PHASE_TRAIN] += model.ns;
+        }
+        (void)hipEventDestroy(model.ev0);
+        (void)hipEventDestroy(model.ev1);
+    }
     if (rc == S512_OK && stats && model.training && ls.pending) {
         // the sort finished within one level: report the loss all the same
         (void)hipEventSynchronize(model.trained);
