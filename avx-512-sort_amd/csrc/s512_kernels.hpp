@@ -902,11 +902,17 @@ inline void launch_count(int shape, uint32_t grid, hipStream_t st, const PartSeg
 // workgroups in all) and NG = 4 otherwise.
 constexpr int CS_SLICE = 64;
 constexpr int CS_NSLICE = BMAX / CS_SLICE;
+// runs (may be null): a sample of the runs the scatter will write, for the
+// full-line scatter's gate: the first slice of buckets of each of the first
+// RUN_SAMPLE_SEGS segments adds its nonzero (chunk, bucket) counts to runs[0]
+// and its keys to runs[1] (one pair of atomics per workgroup).
+constexpr uint32_t RUN_SAMPLE_SEGS = 64;
 template <int NG>
 __global__ __launch_bounds__(NG * 64) void k_colscan(const PartSeg* ps, const uint32_t* nseg, uint32_t* hist,
-                                                     uint64_t* bsize) {
+                                                     uint64_t* bsize, uint32_t* runs_out = nullptr) {
     constexpr int CS_BATCH = NG >= 16 ? 16 : 8;  // loads in flight per lane
     __shared__ uint32_t part[NG][CS_SLICE];
+    __shared__ uint32_t rsum[NG][2];
     const uint32_t lane = threadIdx.x & 63u, grp = threadIdx.x >> 6;
     const uint32_t nwork = *nseg * CS_NSLICE;
     for (uint32_t w = blockIdx.x; w < nwork; w += gridDim.x) {
@@ -919,16 +925,42 @@ __global__ __launch_bounds__(NG * 64) void k_colscan(const PartSeg* ps, const ui
         const uint32_t nc = s.nchunks;
         const uint32_t c0 = (uint32_t)((uint64_t)nc * grp / NG), c1 = (uint32_t)((uint64_t)nc * (grp + 1) / NG);
         uint32_t* h = hist + s.hist_base + bkt;
-        uint32_t sum = 0;
+        uint32_t sum = 0, runs = 0;
         for (uint32_t cb = c0; cb < c1; cb += CS_BATCH) {
             uint32_t v[CS_BATCH];
 #pragma unroll
             for (int k = 0; k < CS_BATCH; ++k) v[k] = (on && cb + k < c1) ? h[(uint64_t)(cb + k) * B] : 0u;
 #pragma unroll
-            for (int k = 0; k < CS_BATCH; ++k) sum += v[k];
+            for (int k = 0; k < CS_BATCH; ++k) {
+                sum += v[k];
+                runs += v[k] ? 1u : 0u;
+            }
+        }
+        const bool sample = runs_out && w % CS_NSLICE == 0 && w / CS_NSLICE < RUN_SAMPLE_SEGS;
+        if (sample) {
+            uint32_t ks = sum;
+#pragma unroll
+            for (int d = 32; d >= 1; d >>= 1) {
+                runs += __shfl_xor(runs, d);
+                ks += __shfl_xor(ks, d);
+            }
+            if (lane == 0) {
+                rsum[grp][0] = runs;
+                rsum[grp][1] = ks;
+            }
         }
         part[grp][lane] = sum;
         __syncthreads();
+        if (sample && threadIdx.x == 0) {
+            uint32_t r = 0, k = 0;
+#pragma unroll
+            for (int q = 0; q < NG; ++q) {
+                r += rsum[q][0];
+                k += rsum[q][1];
+            }
+            atomicAdd(&runs_out[0], r);
+            atomicAdd(&runs_out[1], k);
+        }
         uint32_t run = 0, total = 0;
 #pragma unroll
         for (int q = 0; q < NG; ++q) {
@@ -1007,11 +1039,32 @@ struct ScatterLds {
     uint32_t dcount;
 };
 
+// Choice between the full-line scatter and the plain one, made on the device
+// from the level's runs (nonzero (chunk, bucket) counts, k_colscan): runs of
+// FL_RUN_BYTES and more on average (sorted or reversed input) reach whole
+// lines with the plain scatter too, which measured faster there (2^28 fp64
+// keys, sorted: 1.21 + 0.96 ms for both levels against 1.10 + 1.69); shorter
+// runs take the full-line scatter (uniform: 1.25 + 1.31 against 1.66 + 1.59).
+// The host launches both kernels of a pair; one of them returns at once.
+constexpr uint64_t FL_RUN_BYTES = 3072;
+struct ScatterGate {
+    const uint32_t* runs = nullptr;  // sampled runs and their keys (k_colscan)
+    uint32_t side = 0;               // 0: no gate; 1: runs on short runs; 2: runs on long runs
+    template <class T>
+    __device__ __forceinline__ bool skip() const {
+        if (!side) return false;
+        const bool lng = (uint64_t)runs[0] * (FL_RUN_BYTES / sizeof(T)) <= (uint64_t)runs[1];
+        return side == 1 ? lng : !lng;
+    }
+};
+
 template <class T, bool KV, bool IDS, int P, int E, bool WC, bool LRN = false>
 __global__ __launch_bounds__(P) void k_scatter(const PartSeg* ps, const ChunkRef* cr, T* a, T* b, int32_t* va,
                                                int32_t* vb, const typename KeyTraits<T>::K* trees,
                                                const uint32_t* guides, const uint32_t* hist, const uint64_t* bstart,
-                                               const uint32_t* nchunk, const uint16_t* ids) {
+                                               const uint32_t* nchunk, const uint16_t* ids,
+                                               const ScatterGate gate = ScatterGate{}) {
+    if (gate.template skip<T>()) return;  // the whole workgroup, before any barrier
     using Tr = KeyTraits<T>;
     using K = typename Tr::K;
     using L = ScatterLds<T, KV, IDS, P, E, WC>;
@@ -1270,7 +1323,9 @@ struct ScatterFlLds {
 template <class T, int P, int E>
 __global__ __launch_bounds__(P) void k_scatter_fl(const PartSeg* ps, const ChunkRef* cr, T* a, T* b,
                                                   const uint32_t* hist, const uint64_t* bstart,
-                                                  const uint32_t* nchunk, const uint16_t* ids, uint32_t per_wg) {
+                                                  const uint32_t* nchunk, const uint16_t* ids, uint32_t per_wg,
+                                                  const ScatterGate gate = ScatterGate{}) {
+    if (gate.template skip<T>()) return;  // the whole workgroup, before any barrier
     using L = ScatterFlLds<T, P, E>;
     using Rec = typename L::Rec;
     constexpr int ROUND = L::ROUND, LINE = L::LINE, CW = L::CW;
@@ -1754,7 +1809,7 @@ inline void launch_scatter(int shape, bool use_ids, uint32_t grid, hipStream_t s
                            const ChunkRef* cr, T* a, T* b, int32_t* va, int32_t* vb,
                            const typename KeyTraits<T>::K* trees, const uint32_t* guides, const uint32_t* hist,
                            const uint64_t* bstart, const uint32_t* nchunk,
-                           const uint16_t* ids, bool lrn = false) {
+                           const uint16_t* ids, bool lrn = false, const uint32_t* runs = nullptr) {
 #define S512_SCATTER(I, PP, EE, W)                                                                              \
     k_scatter<T, KV, I, PP, EE, W><<<grid, PP, 0, st>>>(ps, cr, a, b, va, vb, trees, guides, hist, bstart, nchunk, \
                                                         ids)
@@ -1780,11 +1835,16 @@ inline void launch_scatter(int shape, bool use_ids, uint32_t grid, hipStream_t s
     }
     if constexpr (!KV) {
         if (shape == SCATTER_FL && use_ids) {
-            // persistent: about one workgroup per CU, each on a contiguous range of chunks
+            // persistent: about one workgroup per CU, each on a contiguous range of chunks;
+            // with the level's runs counted, paired with the plain scatter (one of the two runs)
             const uint32_t cus = device_cu_count();
             const uint32_t per = (grid + cus - 1) / cus;
+            const ScatterGate g1{runs, runs ? 1u : 0u}, g2{runs, 2u};
             k_scatter_fl<T, 512, 16><<<(grid + per - 1) / per, 512, 0, st>>>(ps, cr, a, b, hist, bstart, nchunk,
-                                                                             ids, per);
+                                                                             ids, per, g1);
+            if (runs)
+                k_scatter<T, false, true, 512, 16, false><<<grid, 512, 0, st>>>(ps, cr, a, b, nullptr, nullptr, trees,
+                                                                               guides, hist, bstart, nchunk, ids, g2);
             return;
         }
         if (shape == SCATTER_LB) {

@@ -454,15 +454,19 @@ s512_status run_levels(T* a, int32_t* va, T* b, int32_t* vb, size_t n, std::vect
         S512_CHECK_LAUNCH();
         pf.end();
         pf.begin(S512_PHASE_SCAN);
-        if (max_nchunks >= 64) k_colscan<16><<<m * CS_NSLICE, 1024, 0, st>>>(d_ps, d_cnt + 12, d_hist, d_bsize);
-        else k_colscan<4><<<m * CS_NSLICE, 256, 0, st>>>(d_ps, d_cnt + 12, d_hist, d_bsize);
+        // d_cnt[15], d_cnt[16]: sampled runs of the level and their keys, for the
+        // full-line scatter's gate
+        uint32_t* d_runs = (!KV && prm.scatter_shape == SCATTER_FL && d_ids) ? d_cnt + 15 : nullptr;
+        if (max_nchunks >= 64)
+            k_colscan<16><<<m * CS_NSLICE, 1024, 0, st>>>(d_ps, d_cnt + 12, d_hist, d_bsize, d_runs);
+        else k_colscan<4><<<m * CS_NSLICE, 256, 0, st>>>(d_ps, d_cnt + 12, d_hist, d_bsize, d_runs);
         S512_CHECK_LAUNCH();
         k_bucket_scan<<<m, BMAX, 0, st>>>(d_ps, d_cnt + 12, d_bsize, d_bstart);
         S512_CHECK_LAUNCH();
         pf.end();
         pf.begin(S512_PHASE_SCATTER);
         launch_scatter<T, KV>(prm.scatter_shape, d_ids != nullptr, tc, st, d_ps, d_cr, a, b, va, vb, d_tree,
-                              d_guide, d_hist, d_bstart, d_cnt + 13, d_ids, any_learned);
+                              d_guide, d_hist, d_bstart, d_cnt + 13, d_ids, any_learned, d_runs);
         S512_CHECK_LAUNCH();
         pf.end();
         pf.begin(S512_PHASE_PLAN);

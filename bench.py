@@ -573,6 +573,9 @@ def _kernel_traffic(traffic, op, prefix):
     names = set()
     for kn, kd in d["kernels"].items():
         short = kn.split("s512::")[-1]
+        # a kernel of a gated pair that returned at once moves no bytes: not a launch of the phase
+        if kd["fetch_per_dispatch"] + kd["write_per_dispatch"] < 1e6:
+            continue
         if short.startswith(prefix):
             f += kd["fetch_per_dispatch"] * kd["dispatches"]
             w += kd["write_per_dispatch"] * kd["dispatches"]
@@ -813,7 +816,10 @@ def _roofline(ph_ms, ph_l, ph_k, key_bytes, traffic_of=None):
         out["traffic_detail"] = traffic
         out["measured_frac"] = traffic["traffic"] / (per_launch_ms / 1e3) / 1e9 / HBM_PEAK_GBS
         if len(traffic.get("kernels", ())) == 1:
-            out["kernel"] = traffic["kernels"][0]  # the kernel the counter runs saw in this phase
+            out["kernel_in_counter_run"] = traffic["kernels"][0]  # the kernel the counter runs saw in this phase
+    if dom == "scatter":
+        out["kernel"] = ("scatter phase: k_scatter_fl or k_scatter per level (a device-side gate picks one of the "
+                         "pair from the level's runs; the other returns at once)")
     return out
 
 
