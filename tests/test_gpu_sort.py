@@ -182,3 +182,47 @@ def test_sort_f64_subnormal_inf_fast_path(s512, orc, dev, n):
     t = to_dev(a, dev)
     s512.sort(t)
     assert np.array_equal(to_host(t).view(np.uint64), want.view(np.uint64))
+
+
+def _presorted(kind, n, rng):
+    """fp64 inputs in order, in reverse, or almost so."""
+    if kind == "sorted":
+        return np.sort(rng.standard_normal(n))
+    if kind == "reversed":
+        return np.sort(rng.standard_normal(n))[::-1].copy()
+    if kind == "reversed_dups":  # non-increasing with runs of equal keys
+        return np.repeat(np.sort(rng.integers(0, n // 7, n // 7 + 1))[::-1], 7)[:n].astype(np.float64)
+    if kind == "sorted_swaps":  # sorted but for a few transpositions (no leaf may skip its network)
+        a = np.sort(rng.standard_normal(n))
+        i = rng.integers(0, n, 64)
+        j = np.minimum(i + rng.integers(1, 50, 64), n - 1)
+        a[i], a[j] = a[j].copy(), a[i].copy()
+        return a
+    if kind == "reversed_swaps":
+        a = np.sort(rng.standard_normal(n))[::-1].copy()
+        i = rng.integers(0, n, 64)
+        j = np.minimum(i + rng.integers(1, 50, 64), n - 1)
+        a[i], a[j] = a[j].copy(), a[i].copy()
+        return a
+    if kind == "sawtooth":  # ascending runs of 1000 keys
+        return np.tile(np.arange(1000, dtype=np.float64), n // 1000 + 1)[:n]
+    if kind == "organ":  # ascending then descending
+        h = np.arange(n // 2, dtype=np.float64)
+        return np.concatenate([h, h[::-1], [0.5] * (n - 2 * (n // 2))])
+    raise ValueError(kind)
+
+
+@pytest.mark.parametrize("kind", ["sorted", "reversed", "reversed_dups", "sorted_swaps", "reversed_swaps",
+                                  "sawtooth", "organ"])
+@pytest.mark.parametrize("n", [1500, 9000, 300_001, 3_000_001])
+def test_sort_f64_runs_in_order(s512, dev, kind, n):
+    """Inputs with order already in them, where the scatter's long runs take
+    the plain scatter (the full-line scatter's gate) and the leaves come out
+    nearly sorted or nearly reversed.  Against numpy's sort bit for bit: whole
+    inputs in order or reversed (with and without equal keys), the same with a
+    few transpositions, and inputs made of runs."""
+    rng = np.random.default_rng(sum(map(ord, kind)) + n)
+    a = _presorted(kind, n, rng)
+    t = to_dev(a, dev)
+    s512.sort(t)
+    assert np.array_equal(to_host(t).view(np.uint64), np.sort(a).view(np.uint64)), kind
