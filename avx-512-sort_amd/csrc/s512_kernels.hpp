@@ -905,14 +905,15 @@ constexpr int CS_NSLICE = BMAX / CS_SLICE;
 // runs (may be null): a sample of the runs the scatter will write, for the
 // full-line scatter's gate: the first slice of buckets of each of the first
 // RUN_SAMPLE_SEGS segments adds its nonzero (chunk, bucket) counts to runs[0]
-// and its keys to runs[1] (one pair of atomics per workgroup).
+// and the (chunk, bucket) pairs it looked at to runs[1] (one pair of atomics
+// per workgroup).
 constexpr uint32_t RUN_SAMPLE_SEGS = 64;
 template <int NG>
 __global__ __launch_bounds__(NG * 64) void k_colscan(const PartSeg* ps, const uint32_t* nseg, uint32_t* hist,
                                                      uint64_t* bsize, uint32_t* runs_out = nullptr) {
     constexpr int CS_BATCH = NG >= 16 ? 16 : 8;  // loads in flight per lane
     __shared__ uint32_t part[NG][CS_SLICE];
-    __shared__ uint32_t rsum[NG][2];
+    __shared__ uint32_t rsum[NG][1];
     const uint32_t lane = threadIdx.x & 63u, grp = threadIdx.x >> 6;
     const uint32_t nwork = *nseg * CS_NSLICE;
     for (uint32_t w = blockIdx.x; w < nwork; w += gridDim.x) {
@@ -938,28 +939,18 @@ __global__ __launch_bounds__(NG * 64) void k_colscan(const PartSeg* ps, const ui
         }
         const bool sample = runs_out && w % CS_NSLICE == 0 && w / CS_NSLICE < RUN_SAMPLE_SEGS;
         if (sample) {
-            uint32_t ks = sum;
 #pragma unroll
-            for (int d = 32; d >= 1; d >>= 1) {
-                runs += __shfl_xor(runs, d);
-                ks += __shfl_xor(ks, d);
-            }
-            if (lane == 0) {
-                rsum[grp][0] = runs;
-                rsum[grp][1] = ks;
-            }
+            for (int d = 32; d >= 1; d >>= 1) runs += __shfl_xor(runs, d);
+            if (lane == 0) rsum[grp][0] = runs;
         }
         part[grp][lane] = sum;
         __syncthreads();
         if (sample && threadIdx.x == 0) {
-            uint32_t r = 0, k = 0;
+            uint32_t r = 0;
 #pragma unroll
-            for (int q = 0; q < NG; ++q) {
-                r += rsum[q][0];
-                k += rsum[q][1];
-            }
+            for (int q = 0; q < NG; ++q) r += rsum[q][0];
             atomicAdd(&runs_out[0], r);
-            atomicAdd(&runs_out[1], k);
+            atomicAdd(&runs_out[1], min(B - b0, (uint32_t)CS_SLICE) * nc);
         }
         uint32_t run = 0, total = 0;
 #pragma unroll
@@ -1040,21 +1031,23 @@ struct ScatterLds {
 };
 
 // Choice between the full-line scatter and the plain one, made on the device
-// from the level's runs (nonzero (chunk, bucket) counts, k_colscan): runs of
-// FL_RUN_BYTES and more on average (sorted or reversed input) reach whole
-// lines with the plain scatter too, which measured faster there (2^28 fp64
-// keys, sorted: 1.21 + 0.96 ms for both levels against 1.10 + 1.69); shorter
-// runs take the full-line scatter (uniform: 1.25 + 1.31 against 1.66 + 1.59).
-// The host launches both kernels of a pair; one of them returns at once.
-constexpr uint64_t FL_RUN_BYTES = 3072;
+// from how many (chunk, bucket) pairs of the level hold keys (sampled by
+// k_colscan).  When few do (sorted or reversed input: a chunk's keys fall in a
+// few neighbouring buckets, in long runs), the plain scatter writes whole lines
+// too and measured faster (2^28 fp64 keys, sorted: 1.21 + 0.96 ms for both
+// levels against 1.10 + 1.69); when a quarter or more do, the full-line
+// scatter (uniform, where half the pairs, the ordinary buckets, hold keys:
+// 1.25 + 1.31 against 1.66 + 1.59; 2^30 mixture keys, runs of 512 keys: 30.1
+// against 32.8 ms for the sort).  The host launches both kernels of a pair;
+// one of them returns at once.
 struct ScatterGate {
-    const uint32_t* runs = nullptr;  // sampled runs and their keys (k_colscan)
-    uint32_t side = 0;               // 0: no gate; 1: runs on short runs; 2: runs on long runs
+    const uint32_t* runs = nullptr;  // sampled nonzero pairs, pairs looked at (k_colscan)
+    uint32_t side = 0;               // 0: no gate; 1: runs on dense pairs; 2: runs on sparse pairs
     template <class T>
     __device__ __forceinline__ bool skip() const {
         if (!side) return false;
-        const bool lng = (uint64_t)runs[0] * (FL_RUN_BYTES / sizeof(T)) <= (uint64_t)runs[1];
-        return side == 1 ? lng : !lng;
+        const bool sparse = (uint64_t)runs[0] * 4 < (uint64_t)runs[1];
+        return side == 1 ? sparse : !sparse;
     }
 };
 
