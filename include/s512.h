@@ -228,7 +228,12 @@ typedef struct s512_options {
                                  lines, a bucket's unfinished line waits in LDS for the
                                  next round; persistent workgroups over contiguous chunk
                                  ranges; stored ids; keys-only sorts, key/value sorts run
-                                 shape 0)                                               */
+                                 shape 0), launched as a pair with shape 0 reading the
+                                 stored ids: per level, a device-side gate runs the
+                                 plain one when under a quarter of the (chunk, bucket)
+                                 pairs hold keys (sorted or reversed input), the
+                                 full-line one otherwise.  The tuned default for
+                                 double keys from 2^27 keys on                          */
     uint32_t count_variant;   /* 0: default (shape 2); else 1 + count shape 0..4 = (threads,
                                  keys per thread per batch, keys classified together):
                                  256x16x8, 256x16x4, 256x8x8, 512x8x8, 256x8x4          */
