@@ -299,8 +299,8 @@ using DefaultOrder = UintOrder<typename KeyTraits<T>::K>;
 
 // Sort every listed segment (size <= P*E) in one workgroup; result goes to the
 // caller's array `a` at the same range.
-template <class T, int P, int E, class Pol = DefaultOrder<T>>
-__global__ __launch_bounds__(P) void k_tile_sort(const TileSeg* segs, const uint32_t* count, T* a, const T* b) {
+template <class T, int P, int E, class Pol>
+__device__ __forceinline__ void tile_sort_body(const TileSeg* segs, const uint32_t* count, T* a, const T* b) {
     using IO = KeyIO<T, Pol>;
     using K = typename Pol::key_t;
     using BB = BlockBitonic<Pol, P, E>;
@@ -328,6 +328,18 @@ __global__ __launch_bounds__(P) void k_tile_sort(const TileSeg* segs, const uint
         }
         __syncthreads();
     }
+}
+template <class T, int P, int E, class Pol = DefaultOrder<T>>
+__global__ __launch_bounds__(P) void k_tile_sort(const TileSeg* segs, const uint32_t* count, T* a, const T* b) {
+    tile_sort_body<T, P, E, Pol>(segs, count, a, b);
+}
+// The same, compiled for three waves per SIMD (a few registers spill): the
+// 4096-key raw-double tile (256 x 16) measured faster this way.
+template <class T, int P, int E, class Pol = DefaultOrder<T>>
+__global__ __launch_bounds__(P) __attribute__((amdgpu_waves_per_eu(3))) void k_tile_sort_w3(const TileSeg* segs,
+                                                                                          const uint32_t* count, T* a,
+                                                                                          const T* b) {
+    tile_sort_body<T, P, E, Pol>(segs, count, a, b);
 }
 
 // Sample sorts of many segments whose sample sizes differ (the segmented
