@@ -763,7 +763,7 @@ EngineParams default_params(bool wide) {
     p.f64_minmax = true;
     p.learned = false;
     p.sample_permille = 10;
-    p.train_steps = 30;
+    p.train_steps = 10;  // over before the first level's scatter starts (30 ran into it)
     p.learn_rate = 2e-3f;
     p.max_loss = 1e-3f;  // RMS CDF error ~3%: the network did not train
     p.imbalance = 4.0f;

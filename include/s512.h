@@ -211,7 +211,7 @@ typedef struct s512_options {
     uint64_t seed;          /* sample positions (0: fixed default)           */
     uint32_t classifier;    /* S512_CLASSIFY_*; learned applies to keys-only int32/double sorts */
     uint32_t sample_permille; /* learned: model sample, per mille of n (0: 10 = 1%) */
-    uint32_t train_steps;   /* learned: Adam steps (0: 30)                  */
+    uint32_t train_steps;   /* learned: Adam steps (0: 10)                  */
     uint32_t imbalance_pct; /* learned: fall back to splitters when a first-level bucket
                                exceeds this percentage of the mean (0: 400) */
     uint32_t scatter_variant; /* 0: default (1); else 1 + shape + 4 * reclassify, where shape
