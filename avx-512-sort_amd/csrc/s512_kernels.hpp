@@ -299,12 +299,12 @@ using DefaultOrder = UintOrder<typename KeyTraits<T>::K>;
 
 // Sort every listed segment (size <= P*E) in one workgroup; result goes to the
 // caller's array `a` at the same range.
-template <class T, int P, int E, class Pol>
+template <class T, int P, int E, class Pol, int LP = default_lane_phases<typename Pol::key_t, E>()>
 __device__ __forceinline__ void tile_sort_body(const TileSeg* segs, const uint32_t* count, T* a, const T* b) {
     using IO = KeyIO<T, Pol>;
     using K = typename Pol::key_t;
-    using BB = BlockBitonic<Pol, P, E>;
-    __shared__ K lds[BB::N];
+    using BB = BlockBitonic<Pol, P, E, LP>;
+    __shared__ K lds[BB::LDS_N];
     const uint32_t cnt = *count;
     const uint32_t t = threadIdx.x;
     for (uint32_t s = blockIdx.x; s < cnt; s += gridDim.x) {
@@ -329,17 +329,18 @@ __device__ __forceinline__ void tile_sort_body(const TileSeg* segs, const uint32
         __syncthreads();
     }
 }
-template <class T, int P, int E, class Pol = DefaultOrder<T>>
+template <class T, int P, int E, class Pol = DefaultOrder<T>, int LP = default_lane_phases<typename Pol::key_t, E>()>
 __global__ __launch_bounds__(P) void k_tile_sort(const TileSeg* segs, const uint32_t* count, T* a, const T* b) {
-    tile_sort_body<T, P, E, Pol>(segs, count, a, b);
+    tile_sort_body<T, P, E, Pol, LP>(segs, count, a, b);
 }
-// The same, compiled for three waves per SIMD (a few registers spill): the
-// 4096-key raw-double tile (256 x 16) measured faster this way.
+// The same, compiled for three waves per SIMD (a few registers spill), with
+// every merge phase through LDS (the lane phases' temporaries spill under that
+// cap): the 4096-key raw-double tile (256 x 16) measured faster this way.
 template <class T, int P, int E, class Pol = DefaultOrder<T>>
 __global__ __launch_bounds__(P) __attribute__((amdgpu_waves_per_eu(3))) void k_tile_sort_w3(const TileSeg* segs,
                                                                                           const uint32_t* count, T* a,
                                                                                           const T* b) {
-    tile_sort_body<T, P, E, Pol>(segs, count, a, b);
+    tile_sort_body<T, P, E, Pol, 0>(segs, count, a, b);
 }
 
 // Sample sorts of many segments whose sample sizes differ (the segmented
@@ -354,7 +355,7 @@ __global__ __launch_bounds__(P) void k_tile_sort_range(const TileSeg* segs, cons
     using IO = KeyIO<T, Pol>;
     using K = typename Pol::key_t;
     using BB = BlockBitonic<Pol, P, E>;
-    __shared__ K lds[BB::N];
+    __shared__ K lds[BB::LDS_N];
     const uint32_t cnt = *count;
     const uint32_t t = threadIdx.x;
     for (uint32_t s = blockIdx.x; s < cnt; s += gridDim.x) {
@@ -419,7 +420,7 @@ template <int P, int E>
 __global__ __launch_bounds__(P) void k_tile_sort_kv(const TileSeg* segs, const uint32_t* count, int32_t* ka,
                                                     int32_t* va, const int32_t* kb, const int32_t* vb) {
     using BB = BlockBitonic<UintOrder<uint64_t>, P, E>;
-    __shared__ uint64_t lds[BB::N];
+    __shared__ uint64_t lds[BB::LDS_N];
     const uint32_t cnt = *count;
     const uint32_t t = threadIdx.x;
     for (uint32_t s = blockIdx.x; s < cnt; s += gridDim.x) {
@@ -470,7 +471,7 @@ __global__ __launch_bounds__(P) void k_tile_sort_kv64(const TileSeg* segs, const
     using Tr = KeyTraits<T>;
     using BB = BlockBitonic<UintOrder<uint64_t>, P, E>;
     constexpr uint32_t N = BB::N;
-    __shared__ uint64_t lds[N];
+    __shared__ uint64_t lds[BB::LDS_N];
     __shared__ uint32_t dup[N];
     __shared__ int32_t lval[N];
     const uint32_t cnt = *count;

@@ -30,7 +30,8 @@ static const uint32_t kCapsKV[] = {256, 512, 1024, 2048, 4096, 8192};         //
 
 // Tile shapes (threads P x keys per thread E) per class and plan.  Plan 0
 // keeps 8 keys per thread up to 2048-key tiles; plan 1 uses 16; plan 2 uses 32
-// (a 2048-key tile is one wave: every stage above the register window is one
+// (a 2048-key tile is one wave: the stages on lane bits of the first three
+// merge phases above the registers are DPP moves, every later window one
 // wave-local LDS round trip, no workgroup barrier).  Every shape's LDS layout
 // is conflict-free (BlockBitonic::phys).
 constexpr int TILE_PLANS = 3;

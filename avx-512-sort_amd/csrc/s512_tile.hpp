@@ -145,8 +145,46 @@ __device__ __forceinline__ void reg_sort(typename Pol::key_t (&v)[E], bool flip_
     reg_low_up<Pol, E, LOGE - 1>(v);
 }
 
+// Ascending stage between lanes l and l^M, every register: the lower lane
+// (upper == false) keeps the min.
+template <class Pol, int E, int M>
+__device__ __forceinline__ void lane_stage_up(typename Pol::key_t (&v)[E], bool upper) {
+    using K = typename Pol::key_t;
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        const K p = xor_lane<M>(v[r]);
+        K lo = v[r], hi = p;
+        Pol::ce(lo, hi);
+        v[r] = upper ? hi : lo;
+    }
+}
+// Stages on lane bits LB, LB-1, .., 0.
+template <class Pol, int E, int LB>
+__device__ __forceinline__ void lane_stages_down(typename Pol::key_t (&v)[E], uint32_t lane) {
+    if constexpr (LB >= 0) {
+        lane_stage_up<Pol, E, (1 << LB)>(v, (lane & (1u << LB)) != 0);
+        lane_stages_down<Pol, E, LB - 1>(v, lane);
+    }
+}
+
 // ---------------------------------------------------------------- workgroup tile
-template <class Pol, int P, int E>
+//
+// Lane phases by default: two.  A third one holds more partner values in
+// registers: the raw-double 2048-key one-wave tile spills with it (1.08
+// against 0.93 ms per 2^28-key sort), the 4-byte 8192-key tile drops to one
+// wave per SIMD.  (The raw-double 1024-key tile takes three: 0.50 against
+// 0.52 ms.)
+template <class K, int E>
+constexpr int default_lane_phases() { return 2; }
+
+// The first LP merge phases above the register bits (LP <= 3) run as lane
+// stages instead of LDS windows.  In the blocked layout index bit LOGE + k is
+// bit k of the thread's logical id; with LP = 3 the logical id is the thread
+// id with bits 2 and 3 swapped, so logical bits 0, 1 and 2 are lane xor 1, 2
+// (DPP quad_perm) and 8 (DPP row_ror:8) -- VALU moves, no LDS.  The swap stays
+// inside every group of 16 lanes, so the lanes that LDS serves together hold
+// the same set of positions as without it.
+template <class Pol, int P, int E, int LP = default_lane_phases<typename Pol::key_t, E>()>
 struct BlockBitonic {
     using K = typename Pol::key_t;
     static constexpr int LOGE = ilog2(E);
@@ -155,22 +193,35 @@ struct BlockBitonic {
     static constexpr int LOGN = LOGE + LOGP;
     static_assert((1 << LOGE) == E && (1 << LOGP) == P, "powers of two");
     static_assert(LOGN >= 2 * LOGE, "window layouts need P >= E");
-    static constexpr size_t LDS_BYTES = sizeof(K) * N;
+    static_assert(LP >= 0 && LP <= 3 && P >= 16, "lane phases");
+
+    __device__ static __forceinline__ uint32_t logical(uint32_t t) {
+        if constexpr (LP >= 3) return (t & ~12u) | ((t & 4u) << 1) | ((t & 8u) >> 1);
+        else return t;
+    }
+    // 8-byte keys, 32 per thread: one pad slot after every 32 (see phys)
+    static constexpr bool PADDED = sizeof(K) == 8 && E == 32;
+    static constexpr int LDS_N = PADDED ? N + N / 32 : N;  // LDS slots
 
     __device__ static __forceinline__ uint32_t idx(uint32_t t, uint32_t r, int w) {
         const uint32_t lm = (1u << w) - 1u;
         return ((t & ~lm) << LOGE) | (r << w) | (t & lm);
     }
-    // LDS position of element i: a GF(2)-linear bijection (each output bit is
-    // XORed only with higher input bits) chosen so that, in every layout the
-    // network uses (striped, blocked and every window) and for every register,
-    // the lanes of a wave touch distinct banks -- both when 32 consecutive
-    // lanes of 8-byte keys (64 lanes of 4-byte keys) are served per cycle and
-    // when 16 (32) are.  Found by exhaustive search over XOR taps; for the
-    // configurations below every access is conflict-free in both models.
+    // LDS position of element i, chosen so that, in every layout the network
+    // uses (striped, blocked and every window) and for every register, the
+    // lanes of a wave touch distinct banks -- both when 32 consecutive lanes
+    // of 8-byte keys (64 lanes of 4-byte keys) are served per cycle and when
+    // 16 (32) are.  8-byte keys with 32 per thread: one pad slot per 32,
+    // i + i/32.  The pad is a sum of the index's bit fields shifted, so in
+    // every layout the position of register r of thread t is base(t) + c(r)
+    // with c(r) a compile-time constant (an instruction offset): no address
+    // arithmetic per key.  Otherwise a GF(2)-linear bijection (each output bit
+    // XORed only with higher input bits), found by exhaustive search over XOR
+    // taps; for the configurations below every access is conflict-free in
+    // both models.
     __device__ static __forceinline__ uint32_t phys(uint32_t i) {
-        if constexpr (sizeof(K) == 8 && E == 32) {
-            return i ^ ((i >> 5) & 31u);
+        if constexpr (PADDED) {
+            return i + (i >> 5);
         } else if constexpr (sizeof(K) == 4 && E == 32) {
             return i ^ ((i >> 5) & 63u);
         } else if constexpr (sizeof(K) == 8 && E == 16) {
@@ -223,37 +274,56 @@ struct BlockBitonic {
         }
     }
 
-    // Merge phase 2^Q: in the blocked layout the direction of element
-    // t*E + r is bit Q of its index, i.e. bit Q-LOGE of t (ascending for the
-    // final phase).  `prev` says whether the registers are flipped for the
-    // previous phase.
-    template <int Q>
-    __device__ static __forceinline__ void merges(K (&v)[E], K* lds, uint32_t t, bool prev) {
-        if constexpr (Q <= LOGN) {
-            const bool cur = (Q < LOGN) && (((t >> (Q - LOGE)) & 1u) != 0);
-            flip_all<Pol, E>(v, prev != cur);
-            windows<Q - 1, 0>(v, lds, t);
-            merges<Q + 1>(v, lds, t, cur);
+    // Lane stages on logical thread bits KB, KB-1, .., 0.
+    template <int KB>
+    __device__ static __forceinline__ void lane_phase(K (&v)[E], uint32_t u) {
+        if constexpr (KB >= 0) {
+            constexpr int M = (LP >= 3 && KB == 2) ? 8 : (1 << KB);
+            lane_stage_up<Pol, E, M>(v, ((u >> KB) & 1u) != 0);
+            lane_phase<KB - 1>(v, u);
         }
     }
 
-    // v holds the blocked layout on entry and on exit; sorted ascending.
+    // Merge phase 2^Q: in the blocked layout the direction of element
+    // u*E + r (u: logical thread) is bit Q of its index, i.e. bit Q-LOGE of u
+    // (ascending for the final phase).  `prev` says whether the registers are
+    // flipped for the previous phase.
+    template <int Q>
+    __device__ static __forceinline__ void merges(K (&v)[E], K* lds, uint32_t u, bool prev) {
+        if constexpr (Q <= LOGN) {
+            const bool cur = (Q < LOGN) && (((u >> (Q - LOGE)) & 1u) != 0);
+            flip_all<Pol, E>(v, prev != cur);
+            if constexpr (Q - LOGE <= LP) {
+                lane_phase<Q - 1 - LOGE>(v, u);
+                reg_low_up<Pol, E, LOGE - 1>(v);
+            } else {
+                windows<Q - 1, 0>(v, lds, u);
+            }
+            merges<Q + 1>(v, lds, u, cur);
+        }
+    }
+
+    // v holds the blocked layout on entry and on exit (thread t holds
+    // elements logical(t)*E ..); sorted ascending.
     __device__ static void sort(K (&v)[E], K* lds, uint32_t t) {
-        const bool f = (t & 1u) != 0;  // direction of merge size E
+        const uint32_t u = logical(t);
+        const bool f = (u & 1u) != 0;  // direction of merge size E
         reg_sort<Pol, E>(v, f);
-        merges<LOGE + 1>(v, lds, t, f);
+        merges<LOGE + 1>(v, lds, u, f);
     }
 
     // striped global -> blocked registers, through LDS
     __device__ static __forceinline__ void from_striped(K (&v)[E], K* lds, uint32_t t) {
+        const uint32_t u = logical(t);
         __syncthreads();
 #pragma unroll
-        for (int r = 0; r < E; ++r) v[r] = lds[phys(t * E + r)];
+        for (int r = 0; r < E; ++r) v[r] = lds[phys(u * E + r)];
         __syncthreads();
     }
     __device__ static __forceinline__ void to_striped(const K (&v)[E], K* lds, uint32_t t) {
+        const uint32_t u = logical(t);
 #pragma unroll
-        for (int r = 0; r < E; ++r) lds[phys(t * E + r)] = v[r];
+        for (int r = 0; r < E; ++r) lds[phys(u * E + r)] = v[r];
         __syncthreads();
     }
 };
@@ -266,31 +336,12 @@ struct WaveBitonic {
     static constexpr int N = 64 * E;
     static constexpr int LOGN = LOGE + 6;
 
-    // ascending stage between lanes l and l^M: the lower lane keeps the min
-    template <int M>
-    __device__ static __forceinline__ void lane_stage(K (&v)[E], uint32_t lane) {
-        const bool upper = (lane & M) != 0;
-#pragma unroll
-        for (int r = 0; r < E; ++r) {
-            const K p = xor_lane<M>(v[r]);
-            K lo = v[r], hi = p;
-            Pol::ce(lo, hi);
-            v[r] = upper ? hi : lo;
-        }
-    }
-    template <int LB>
-    __device__ static __forceinline__ void lane_stages_down(K (&v)[E], uint32_t lane) {
-        if constexpr (LB >= 0) {
-            lane_stage<(1 << LB)>(v, lane);
-            lane_stages_down<LB - 1>(v, lane);
-        }
-    }
     template <int Q>
     __device__ static __forceinline__ void merges(K (&v)[E], uint32_t lane, bool prev) {
         if constexpr (Q <= LOGN) {
             const bool cur = (Q < LOGN) && (((lane >> (Q - LOGE)) & 1u) != 0);
             flip_all<Pol, E>(v, prev != cur);
-            lane_stages_down<Q - 1 - LOGE>(v, lane);
+            lane_stages_down<Pol, E, Q - 1 - LOGE>(v, lane);
             reg_low_up<Pol, E, LOGE - 1>(v);
             merges<Q + 1>(v, lane, cur);
         }
