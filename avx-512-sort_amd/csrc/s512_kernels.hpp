@@ -333,15 +333,6 @@ template <class T, int P, int E, class Pol = DefaultOrder<T>, int LP = default_l
 __global__ __launch_bounds__(P) void k_tile_sort(const TileSeg* segs, const uint32_t* count, T* a, const T* b) {
     tile_sort_body<T, P, E, Pol, LP>(segs, count, a, b);
 }
-// The same, compiled for three waves per SIMD (a few registers spill), with
-// every merge phase through LDS (the lane phases' temporaries spill under that
-// cap): the 4096-key raw-double tile (256 x 16) measured faster this way.
-template <class T, int P, int E, class Pol = DefaultOrder<T>>
-__global__ __launch_bounds__(P) __attribute__((amdgpu_waves_per_eu(3))) void k_tile_sort_w3(const TileSeg* segs,
-                                                                                          const uint32_t* count, T* a,
-                                                                                          const T* b) {
-    tile_sort_body<T, P, E, Pol, 0>(segs, count, a, b);
-}
 
 // Sample sorts of many segments whose sample sizes differ (the segmented
 // sort's large segments: 16 F samples for fan-outs F from 2 to 512).  One

@@ -48,6 +48,7 @@ __device__ __forceinline__ void cex_up(K& a, K& b) {
 template <class K>
 struct UintOrder {  // encoded keys: bitwise NOT reverses the unsigned order
     using key_t = K;
+    static constexpr bool SIGNED_LANES = false;
     __device__ static __forceinline__ void ce(K& a, K& b) { cex_up(a, b); }
     __device__ static __forceinline__ K flip(K x, bool f) { return f ? (K)~x : x; }
     __device__ static __forceinline__ bool less(K a, K b) { return a < b; }
@@ -58,6 +59,9 @@ struct UintOrder {  // encoded keys: bitwise NOT reverses the unsigned order
 // lower to single v_min_f64 / v_max_f64 instructions.
 struct F64Order {
     using key_t = double;
+    // lane stages on negated keys (BlockBitonic::signed_lane_phase)
+    static constexpr bool SIGNED_LANES = true;
+    __device__ static __forceinline__ double min_neg(double a, double b) { return __builtin_fmin(a, -b); }
     __device__ static __forceinline__ void ce(double& a, double& b) {
         const double lo = __builtin_fmin(a, b), hi = __builtin_fmax(a, b);
         a = lo;
@@ -170,10 +174,8 @@ __device__ __forceinline__ void lane_stages_down(typename Pol::key_t (&v)[E], ui
 // ---------------------------------------------------------------- workgroup tile
 //
 // Lane phases by default: two.  A third one holds more partner values in
-// registers: the raw-double 2048-key one-wave tile spills with it (1.08
-// against 0.93 ms per 2^28-key sort), the 4-byte 8192-key tile drops to one
-// wave per SIMD.  (The raw-double 1024-key tile takes three: 0.50 against
-// 0.52 ms.)
+// registers (the 4-byte 8192-key tile drops to one wave per SIMD with it);
+// the raw-double tiles, one min per key and lane stage, choose per shape.
 template <class K, int E>
 constexpr int default_lane_phases() { return 2; }
 
@@ -261,6 +263,9 @@ struct BlockBitonic {
 
     // All stages of one merge phase above the register bits, window by window
     // from the top bit down, then back to the blocked layout for the rest.
+    // (Ending the windows in lane stages once the bits left above the
+    // registers are lane bits saves a round trip in some phases, but moved the
+    // register allocation and measured within 1 %.)
     template <int HI, int WPREV>
     __device__ static __forceinline__ void windows(K (&v)[E], K* lds, uint32_t t) {
         if constexpr (HI >= LOGE) {
@@ -284,6 +289,42 @@ struct BlockBitonic {
         }
     }
 
+    // The same for raw doubles, one min per key and stage: during the stage
+    // on bit KB the keys of the threads whose bit KB is set (the upper side,
+    // which keeps the max) are held negated, and max(a, b) = -min(-a, -b), so
+    // both sides take min(own, -partner).  Entered with the keys negated for
+    // bit KB; between stages only the threads whose side changes flip, and
+    // after bit 0 the keys are plain again.  Both operands of a min are on
+    // the same side, so no +0.0 meets a -0.0.
+    template <int KB>
+    __device__ static __forceinline__ void signed_lane_phase(K (&v)[E], uint32_t u) {
+        if constexpr (KB >= 0) {
+            constexpr int M = (LP >= 3 && KB == 2) ? 8 : (1 << KB);
+#pragma unroll
+            for (int r = 0; r < E; ++r) v[r] = Pol::min_neg(v[r], xor_lane<M>(v[r]));
+            const bool here = ((u >> KB) & 1u) != 0;
+            const bool next = KB > 0 && ((u >> (KB > 0 ? KB - 1 : 0)) & 1u) != 0;
+            flip_all<Pol, E>(v, here != next);
+            signed_lane_phase<KB - 1>(v, u);
+        }
+    }
+
+    // Blocked layout: the stages on index bits HI..LOGE as lane stages
+    // (logical thread bits HI-LOGE..0), then the register bits.  `flip` is a
+    // pending flip of every register, folded into the lane stages' own.
+    template <int HI>
+    __device__ static __forceinline__ void lane_then_regs(K (&v)[E], uint32_t u, bool flip) {
+        constexpr int KB = HI - LOGE;
+        if constexpr (Pol::SIGNED_LANES) {
+            flip_all<Pol, E>(v, flip != (((u >> KB) & 1u) != 0));
+            signed_lane_phase<KB>(v, u);
+        } else {
+            flip_all<Pol, E>(v, flip);
+            lane_phase<KB>(v, u);
+        }
+        reg_low_up<Pol, E, LOGE - 1>(v);
+    }
+
     // Merge phase 2^Q: in the blocked layout the direction of element
     // u*E + r (u: logical thread) is bit Q of its index, i.e. bit Q-LOGE of u
     // (ascending for the final phase).  `prev` says whether the registers are
@@ -292,11 +333,10 @@ struct BlockBitonic {
     __device__ static __forceinline__ void merges(K (&v)[E], K* lds, uint32_t u, bool prev) {
         if constexpr (Q <= LOGN) {
             const bool cur = (Q < LOGN) && (((u >> (Q - LOGE)) & 1u) != 0);
-            flip_all<Pol, E>(v, prev != cur);
             if constexpr (Q - LOGE <= LP) {
-                lane_phase<Q - 1 - LOGE>(v, u);
-                reg_low_up<Pol, E, LOGE - 1>(v);
+                lane_then_regs<Q - 1>(v, u, prev != cur);
             } else {
+                flip_all<Pol, E>(v, prev != cur);
                 windows<Q - 1, 0>(v, lds, u);
             }
             merges<Q + 1>(v, lds, u, cur);

@@ -18,8 +18,7 @@ void launch_tile_f64_fast(int cls, int plan, const TileSeg* segs, const uint32_t
     if (count == 0) return;
     const uint32_t gw = (count + 3) / 4;
 #define S512_TF(PP, EE) k_tile_sort<double, PP, EE, F64Order><<<count, PP, 0, st>>>(segs, dcount, a, b)
-#define S512_TF3(PP, EE) k_tile_sort_w3<double, PP, EE, F64Order><<<count, PP, 0, st>>>(segs, dcount, a, b)
-#define S512_TFL3(PP, EE) k_tile_sort<double, PP, EE, F64Order, 3><<<count, PP, 0, st>>>(segs, dcount, a, b)
+#define S512_TFL(PP, EE, LP) k_tile_sort<double, PP, EE, F64Order, LP><<<count, PP, 0, st>>>(segs, dcount, a, b)
     if (cls == 0) { k_wave_sort<double, 4, F64Order><<<gw, 256, 0, st>>>(segs, dcount, a, b); return; }
     switch (plan * 8 + cls) {
         case 1: S512_TF(128, 4); break;
@@ -30,22 +29,25 @@ void launch_tile_f64_fast(int cls, int plan, const TileSeg* segs, const uint32_t
         case 8 + 2: S512_TF(64, 16); break;
         case 8 + 3: S512_TF(128, 16); break;
         case 8 + 4: S512_TF(256, 16); break;
-        case 16 + 1: S512_TF(64, 8); break;
-        case 16 + 2: S512_TFL3(64, 16); break;  // three lane phases: 0.50 against 0.52 ms (2^28)
-        case 16 + 3: S512_TF(64, 32); break;
         // plan 2 keeps 32 keys per thread up to the 2048-key class only: its
         // 4096- and 8192-key tiles measured faster with 16 (2^28 keys, uniform and
-        // normal: 115 against 140 us, 24 against 32 us per launch)
-        case 16 + 4: S512_TF3(256, 16); break;  // three waves per SIMD: 109 against 115-124 us (2^28),
-                                                 // 3.6 against 4.1 ms (2^30)
-        case 16 + 5: S512_TF(512, 16); break;    // three waves measured slower here (10.7 against 10.5 ms at 2^30)
+        // normal: 115 against 140 us, 24 against 32 us per launch).  Lane phases
+        // per shape (per launch): three for 512-2048 and 8192 keys (2048 keys at
+        // 2^28: 897 against 924 us with two), two for 4096 keys, whose 160
+        // registers leave room for three waves per SIMD (2^28: 94 against 115 us
+        // for the tile with every phase through LDS, built for three waves with
+        // spills; 2^30: 1.86 against 2.17 ms)
+        case 16 + 1: S512_TFL(64, 8, 3); break;
+        case 16 + 2: S512_TFL(64, 16, 3); break;
+        case 16 + 3: S512_TFL(64, 32, 3); break;
+        case 16 + 4: S512_TFL(256, 16, 2); break;
+        case 16 + 5: S512_TFL(512, 16, 3); break;
         case 6:
         case 8 + 6: S512_TF(1024, 16); break;
         case 16 + 6: S512_TF(512, 32); break;
         default: S512_TF(512, 16); break;
     }
-#undef S512_TFL3
-#undef S512_TF3
+#undef S512_TFL
 #undef S512_TF
 }
 
