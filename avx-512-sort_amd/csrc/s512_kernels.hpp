@@ -334,6 +334,18 @@ __global__ __launch_bounds__(P) void k_tile_sort(const TileSeg* segs, const uint
     tile_sort_body<T, P, E, Pol, LP>(segs, count, a, b);
 }
 
+// The same with a floor of W waves per SIMD (registers capped at 512 / W, some
+// spill to scratch): for tiles whose LDS holds more workgroups per CU than
+// their registers would.  The 8192-key raw-double tile (512 x 16, 64 KiB of
+// LDS) runs two workgroups per CU this way instead of one: 5.41 against
+// 6.13 ms per 2^30-key sort.
+template <class T, int P, int E, class Pol, int LP, int W>
+__global__ __launch_bounds__(P) __attribute__((amdgpu_waves_per_eu(W))) void k_tile_sort_wpe(const TileSeg* segs,
+                                                                                           const uint32_t* count,
+                                                                                           T* a, const T* b) {
+    tile_sort_body<T, P, E, Pol, LP>(segs, count, a, b);
+}
+
 // Sample sorts of many segments whose sample sizes differ (the segmented
 // sort's large segments: 16 F samples for fan-outs F from 2 to 512).  One
 // launch per network size P*E sorts the listed segments with lo < size <=

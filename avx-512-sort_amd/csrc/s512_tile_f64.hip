@@ -19,6 +19,8 @@ void launch_tile_f64_fast(int cls, int plan, const TileSeg* segs, const uint32_t
     const uint32_t gw = (count + 3) / 4;
 #define S512_TF(PP, EE) k_tile_sort<double, PP, EE, F64Order><<<count, PP, 0, st>>>(segs, dcount, a, b)
 #define S512_TFL(PP, EE, LP) k_tile_sort<double, PP, EE, F64Order, LP><<<count, PP, 0, st>>>(segs, dcount, a, b)
+#define S512_TFW(PP, EE, LP, W) \
+    k_tile_sort_wpe<double, PP, EE, F64Order, LP, W><<<count, PP, 0, st>>>(segs, dcount, a, b)
     if (cls == 0) { k_wave_sort<double, 4, F64Order><<<gw, 256, 0, st>>>(segs, dcount, a, b); return; }
     switch (plan * 8 + cls) {
         case 1: S512_TF(128, 4); break;
@@ -32,21 +34,22 @@ void launch_tile_f64_fast(int cls, int plan, const TileSeg* segs, const uint32_t
         // plan 2 keeps 32 keys per thread up to the 2048-key class only: its
         // 4096- and 8192-key tiles measured faster with 16 (2^28 keys, uniform and
         // normal: 115 against 140 us, 24 against 32 us per launch).  Lane phases
-        // per shape (per launch): three for 512-2048 and 8192 keys (2048 keys at
-        // 2^28: 897 against 924 us with two), two for 4096 keys, whose 160
+        // per shape (per launch): three for 512-2048 keys (2048 keys at 2^28:
+        // 897 against 924 us with two), two for 4096 and 8192 keys; 4096 keys: 160
         // registers leave room for three waves per SIMD (2^28: 94 against 115 us
         // for the tile with every phase through LDS, built for three waves with
         // spills; 2^30: 1.86 against 2.17 ms)
         case 16 + 1: S512_TFL(64, 8, 3); break;
         case 16 + 2: S512_TFL(64, 16, 3); break;
         case 16 + 3: S512_TFL(64, 32, 3); break;
-        case 16 + 4: S512_TFL(256, 16, 2); break;
-        case 16 + 5: S512_TFL(512, 16, 3); break;
+        case 16 + 4: S512_TFL(256, 16, 2); break;  // capped at four waves: 1.94 against 1.88 ms (2^30)
+        case 16 + 5: S512_TFW(512, 16, 2, 4); break;  // two workgroups per CU (k_tile_sort_wpe)
         case 6:
         case 8 + 6: S512_TF(1024, 16); break;
         case 16 + 6: S512_TF(512, 32); break;
         default: S512_TF(512, 16); break;
     }
+#undef S512_TFW
 #undef S512_TFL
 #undef S512_TF
 }
