@@ -189,6 +189,20 @@ def test_tile_16k_class_after_partition(s512, orc, dev, dtype, variant):
         assert np.array_equal(got, np.sort(a)), f"tile_variant={variant}"
 
 
+@pytest.mark.parametrize("variant", range(1, 4))
+@pytest.mark.parametrize("logf", [3, 4, 5, 6, 7])
+def test_fp64_minmax_leaf_classes_after_partition(s512, orc, dev, variant, logf):
+    """One partition level of 100,000 normal doubles with fan-out 2^logf
+    leaves buckets of about 12,500, 6,250, 3,125, 1,560 and 780 keys: every
+    raw-double leaf shape from 512 to 16384 keys (with lane phases, the
+    four-wave 8192-key tile) sorts leaves of a partition, bit-exact against
+    numpy's sort."""
+    a = orc.gen_f64("normal", 100_000, 17 + logf)
+    got, st = _run(s512, dev, a, tile_variant=variant, max_fanout_log2=logf)
+    assert st.levels >= 1, st.levels
+    assert np.array_equal(got.view(np.uint64), np.sort(a).view(np.uint64)), (variant, logf)
+
+
 def test_levels_with_thousands_of_next_segments(s512, dev):
     """Fan-out 2 on 268,435,456 int32 keys: about 14 partition levels, with
     8,192 segments handed to the last one (more than the 4,096 that come back
