@@ -734,6 +734,14 @@ __device__ __forceinline__ bool nan_or_negzero(uint64_t k) {
     return k == 0x7FFFFFFFFFFFFFFFull || k > 0xFFF0000000000000ull || k < 0x000FFFFFFFFFFFFFull;
 }
 __device__ __forceinline__ bool nan_or_negzero(uint32_t) { return false; }
+// The same test on the raw bits of a key of type T: for doubles one
+// v_cmp_class_f64 with the classes signalling NaN, quiet NaN and negative zero
+// (three 64-bit compares on the encoded key otherwise); other types never.
+template <class T, class K>
+__device__ __forceinline__ bool nan_or_negzero_raw(K raw) {
+    if constexpr (std::is_same<T, double>::value) return __builtin_amdgcn_class(__builtin_bit_cast(double, raw), 0x23);
+    else return false;
+}
 
 // Histogram and rank helpers over the E keys a thread holds (all lanes of the
 // wave active).  Every lane adds one to its key's bucket.  At a key position
@@ -837,7 +845,7 @@ __global__ __launch_bounds__(CNT_P) void k_count(const PartSeg* ps, const ChunkR
             wave_hist_add_many<CNT_U>(lh, dummy, bk);
             if (special) {
 #pragma unroll
-                for (int u = 0; u < CNT_U; ++u) odd |= nan_or_negzero(x[u]);
+                for (int u = 0; u < CNT_U; ++u) odd |= nan_or_negzero_raw<T>(xr[u]);
             }
             if (oid) {
 #pragma unroll
@@ -849,7 +857,7 @@ __global__ __launch_bounds__(CNT_P) void k_count(const PartSeg* ps, const ChunkR
                 const uint64_t i = base + (uint64_t)u * CNT_P + t;
                 if (i < hi) {
                     const uint32_t bk = classify_key<LRN>(s, lsrt, lguide, g, x[u]);
-                    if (special) odd |= nan_or_negzero(x[u]);
+                    if (special) odd |= nan_or_negzero_raw<T>(xr[u]);
                     atomicAdd(&lh[bk], 1u);
                     if (oid) oid[i] = (uint16_t)bk;
                 }

@@ -169,6 +169,22 @@ def test_sort_f64_nan_negzero_fallback(s512, orc, dev, n):
     assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
 
 
+@pytest.mark.parametrize("bits", [0x8000000000000000, 0x7FF8000000000000, 0xFFF8000000000001, 0x7FF0000000000001])
+def test_sort_f64_single_special_key(s512, orc, dev, bits):
+    """One -0.0, quiet NaN, negative NaN or signalling NaN among 300,000 keys
+    (a partition level; the first count pass tests every key's class) is
+    enough to keep the min/max leaves off: bit-exact total order.  A +0.0
+    beside it makes the -0.0 case matter (-0.0 sorts first)."""
+    a = orc.gen_f64("normal", 300_000, 23)
+    a[98_765] = 0.0
+    a.view(np.uint64)[123_457] = np.uint64(bits)
+    want = _enc_sorted_f64(a.copy())
+    t = to_dev(a, dev)
+    st = s512.sort_ex(t)
+    assert st.levels >= 1
+    assert np.array_equal(to_host(t).view(np.uint64), want.view(np.uint64)), hex(bits)
+
+
 @pytest.mark.parametrize("n", [50_000, 5_000_000])
 def test_sort_f64_subnormal_inf_fast_path(s512, orc, dev, n):
     """No NaN / -0.0, so the v_min_f64/v_max_f64 leaves run: subnormals,
