@@ -299,7 +299,7 @@ using DefaultOrder = UintOrder<typename KeyTraits<T>::K>;
 
 // Sort every listed segment (size <= P*E) in one workgroup; result goes to the
 // caller's array `a` at the same range.
-template <class T, int P, int E, class Pol, int LP = default_lane_phases<typename Pol::key_t, E>()>
+template <class T, int P, int E, class Pol, int LP = LANE_PHASES>
 __device__ __forceinline__ void tile_sort_body(const TileSeg* segs, const uint32_t* count, T* a, const T* b) {
     using IO = KeyIO<T, Pol>;
     using K = typename Pol::key_t;
@@ -329,7 +329,7 @@ __device__ __forceinline__ void tile_sort_body(const TileSeg* segs, const uint32
         __syncthreads();
     }
 }
-template <class T, int P, int E, class Pol = DefaultOrder<T>, int LP = default_lane_phases<typename Pol::key_t, E>()>
+template <class T, int P, int E, class Pol = DefaultOrder<T>, int LP = LANE_PHASES>
 __global__ __launch_bounds__(P) void k_tile_sort(const TileSeg* segs, const uint32_t* count, T* a, const T* b) {
     tile_sort_body<T, P, E, Pol, LP>(segs, count, a, b);
 }

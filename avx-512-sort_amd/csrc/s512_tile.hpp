@@ -176,8 +176,7 @@ __device__ __forceinline__ void lane_stages_down(typename Pol::key_t (&v)[E], ui
 // Lane phases by default: two.  A third one holds more partner values in
 // registers (the 4-byte 8192-key tile drops to one wave per SIMD with it);
 // the raw-double tiles, one min per key and lane stage, choose per shape.
-template <class K, int E>
-constexpr int default_lane_phases() { return 2; }
+constexpr int LANE_PHASES = 2;
 
 // The first LP merge phases above the register bits (LP <= 3) run as lane
 // stages instead of LDS windows.  In the blocked layout index bit LOGE + k is
@@ -186,7 +185,7 @@ constexpr int default_lane_phases() { return 2; }
 // (DPP quad_perm) and 8 (DPP row_ror:8) -- VALU moves, no LDS.  The swap stays
 // inside every group of 16 lanes, so the lanes that LDS serves together hold
 // the same set of positions as without it.
-template <class Pol, int P, int E, int LP = default_lane_phases<typename Pol::key_t, E>()>
+template <class Pol, int P, int E, int LP = LANE_PHASES>
 struct BlockBitonic {
     using K = typename Pol::key_t;
     static constexpr int LOGE = ilog2(E);
